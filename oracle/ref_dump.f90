@@ -1,0 +1,108 @@
+!------------------------------------------------------------------------------
+! ref_dump -- driver (project code) around the reference's own Fortran modules
+! s2dw_types_mod, s2dw_error_mod, s2dw_dl_mod and s2dw_core_mod, compiled from
+! the reference sources by oracle/Makefile into oracle/_ref/.  TEST
+! INFRASTRUCTURE ONLY: it writes golden vectors of the reference's setup
+! routines and Wigner planes for tests/golden/.  It never calls the reference's
+! analysis/synthesis routines (they need FFTW, which is absent here).
+!
+! usage (all output is little-endian stream binary to <out>):
+!   ref_dump kernels B alpha J out    -> J_max, bl_scoeff, pass, K_gamma, Phi2, admiss
+!   ref_dump direc   B N out          -> Slm (complex, B*N)
+!   ref_dump dl      l out            -> dl(-l:l,-l:l) at beta = pi/2
+!   ref_dump dlbeta  l beta out       -> dl(-l:l,-l:l) at beta
+!   ref_dump bands   B alpha J out    -> J_max, then (hi, lo) per scale
+!------------------------------------------------------------------------------
+program ref_dump
+
+  use s2dw_types_mod
+  use s2dw_core_mod
+  use s2dw_dl_mod
+
+  implicit none
+
+  character(len=256) :: mode, arg, outf
+  integer :: B, J, N, l, bl_scoeff, J_max, u, jj, bl_hi, bl_lo
+  real(dp) :: alpha, beta
+  real(dp), allocatable :: K_gamma(:,:), Phi2(:), admiss(:), dl(:,:)
+  complex(dpc), allocatable :: Slm(:,:)
+  logical :: pass
+  integer :: ipass
+
+  call get_command_argument(1, mode)
+
+  select case (trim(mode))
+
+  case ('kernels')
+     call get_command_argument(2, arg); read(arg,*) B
+     call get_command_argument(3, arg); read(arg,*) alpha
+     call get_command_argument(4, arg); read(arg,*) J
+     call get_command_argument(5, outf)
+     J_max = s2dw_core_comp_Jmax(B, alpha)
+     allocate(K_gamma(0:J,0:B-1), Phi2(0:B-1), admiss(0:B-1))
+     call s2dw_core_init_kernels(K_gamma, Phi2, bl_scoeff, J, B, alpha)
+     pass = s2dw_core_admiss(admiss, K_gamma, Phi2, B, J)
+     ipass = 0
+     if (pass) ipass = 1
+     open(newunit=u, file=trim(outf), access='stream', form='unformatted', status='replace')
+     write(u) J_max, bl_scoeff, ipass
+     write(u) K_gamma, Phi2, admiss
+     close(u)
+
+  case ('direc')
+     call get_command_argument(2, arg); read(arg,*) B
+     call get_command_argument(3, arg); read(arg,*) N
+     call get_command_argument(4, outf)
+     allocate(Slm(0:B-1,0:N-1))
+     call s2dw_core_init_directionality(Slm, B, N)
+     open(newunit=u, file=trim(outf), access='stream', form='unformatted', status='replace')
+     write(u) Slm
+     close(u)
+
+  case ('dl', 'dlbeta')
+     call get_command_argument(2, arg); read(arg,*) l
+     beta = PION2
+     if (trim(mode) == 'dlbeta') then
+        call get_command_argument(3, arg); read(arg,*) beta
+        call get_command_argument(4, outf)
+     else
+        call get_command_argument(3, outf)
+     end if
+     allocate(dl(-l:l,-l:l))
+     call s2dw_dl_beta_operator(dl, beta, l)
+     open(newunit=u, file=trim(outf), access='stream', form='unformatted', status='replace')
+     write(u) dl
+     close(u)
+
+  case ('bands')
+     call get_command_argument(2, arg); read(arg,*) B
+     call get_command_argument(3, arg); read(arg,*) alpha
+     call get_command_argument(4, arg); read(arg,*) J
+     call get_command_argument(5, outf)
+     J_max = s2dw_core_comp_Jmax(B, alpha)
+     open(newunit=u, file=trim(outf), access='stream', form='unformatted', status='replace')
+     write(u) J_max
+     ! Band limits exactly as written in s2dw_core_analysis_flm2wav_dynamic
+     ! (s2dw_core_mod.f90:606-616), through the public assimilate_int.
+     do jj = 0, J
+        if (s2dw_core_assimilate_int(B / (alpha**(jj-1)), bl_hi)) then
+           bl_hi = min(bl_hi, B)
+        else
+           bl_hi = min(ceiling(B / (alpha**(jj-1))), B)
+        end if
+        if (s2dw_core_assimilate_int(B / (alpha**(jj+1)), bl_lo)) then
+           bl_lo = max(bl_lo, 0)
+        else
+           bl_lo = max(floor(B / (alpha**(jj+1))), 0)
+        end if
+        write(u) bl_hi, bl_lo
+     end do
+     close(u)
+
+  case default
+     write(*,'(a)') 'ref_dump: unknown mode'
+     stop 1
+
+  end select
+
+end program ref_dump

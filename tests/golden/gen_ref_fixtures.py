@@ -1,0 +1,108 @@
+"""Write golden vectors of the reference's own setup routines and Wigner planes.
+
+Runs oracle/_ref/ref_dump (the reference's s2dw_core_mod / s2dw_dl_mod compiled
+from its Fortran sources by ``make -C oracle ref``) and stores its outputs as
+small .npz fixtures beside this script.  Run in a container that holds the
+reference sources; the fixtures are committed, the GPU box only reads them.
+
+    python tests/golden/gen_ref_fixtures.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+
+# (B, alpha, J); J=None means J_max.  Covers the test config (64,2,full J),
+# L=256 N=5 full J, L=512 J_min=2 (J=7), alpha=3 and non-power-of-alpha B.
+KERNEL_CASES = [(8, 2.0, None), (16, 2.0, None), (64, 2.0, None), (64, 2.0, 3), (64, 3.0, None),
+                (100, 2.0, None), (37, 3.0, 2), (81, 3.0, None), (256, 2.0, None), (512, 2.0, 7),
+                (1024, 2.0, None), (200, 3.0, None)]
+DIREC_CASES = [(16, 1), (16, 2), (16, 3), (16, 4), (16, 5), (16, 8), (32, 7), (12, 12)]
+DL_FULL = [1, 2, 3, 4, 5, 7, 8, 15, 16, 31, 32, 64]
+DL_BIG = [100, 127, 255, 256, 511, 1023]
+
+
+def run(*args) -> bytes:
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "o.bin")
+        subprocess.run([REF_DUMP, *map(str, args), out], check=True, capture_output=True)
+        with open(out, "rb") as fh:
+            return fh.read()
+
+
+def jmax_of(B, alpha):
+    b = run("bands", B, alpha, 1)
+    return int(np.frombuffer(b[:4], dtype="<i4")[0])
+
+
+def main():
+    if not os.path.exists(REF_DUMP):
+        raise SystemExit("build the reference driver first: make -C oracle ref")
+    # --- kernels / admissibility / band limits
+    kern = {}
+    for B, alpha, J in KERNEL_CASES:
+        Jm = jmax_of(B, alpha)
+        J = Jm if J is None else J
+        b = run("kernels", B, alpha, J)
+        jmax, bl, ok = np.frombuffer(b[:12], dtype="<i4")
+        d = np.frombuffer(b[12:], dtype="<f8")
+        nK = (J + 1) * B
+        key = f"B{B}_a{int(alpha)}_J{J}"
+        kern[key + "_K"] = d[:nK].reshape((J + 1, B), order="F")
+        kern[key + "_Phi2"] = d[nK:nK + B]
+        kern[key + "_admiss"] = d[nK + B:nK + 2 * B]
+        bb = np.frombuffer(run("bands", B, alpha, J), dtype="<i4")
+        kern[key + "_meta"] = np.array([B, J, int(jmax), int(bl), int(ok)], dtype=np.int64)
+        kern[key + "_alpha"] = np.array([alpha])
+        kern[key + "_bands"] = bb[1:].reshape(J + 1, 2).astype(np.int64)
+    np.savez_compressed(os.path.join(HERE, "ref_kernels.npz"), **kern)
+    # --- J_max over a sweep
+    sweep = []
+    for alpha in (2.0, 3.0, 1.5, 2.5):
+        for B in (2, 3, 4, 5, 7, 8, 9, 16, 27, 31, 32, 33, 63, 64, 65, 81, 100, 128, 243, 255, 256, 512,
+                  729, 1000, 1024):
+            sweep.append((B, alpha, jmax_of(B, alpha)))
+    np.save(os.path.join(HERE, "ref_jmax.npy"), np.array(sweep, dtype=np.float64))
+    # --- directionality
+    direc = {}
+    for B, N in DIREC_CASES:
+        b = run("direc", B, N)
+        direc[f"B{B}_N{N}"] = np.frombuffer(b, dtype="<c16").reshape((B, N), order="F")
+    np.savez_compressed(os.path.join(HERE, "ref_direc.npz"), **direc)
+    # --- Wigner planes at beta = pi/2
+    dls = {}
+    for l in DL_FULL:
+        b = run("dl", l)
+        dls[f"l{l}"] = np.frombuffer(b, dtype="<f8").reshape((2 * l + 1, 2 * l + 1), order="F")
+    rng = np.random.default_rng(7)
+    for l in DL_BIG:
+        b = run("dl", l)
+        P = np.frombuffer(b, dtype="<f8").reshape((2 * l + 1, 2 * l + 1), order="F")
+        Q = P[l:, l:]  # quadrant m', m >= 0: Q[m', m] = d^l_{m' m}
+        dls[f"big{l}_cols"] = np.ascontiguousarray(Q[:, :9])          # m < 9, all m'
+        rows = sorted({0, 1, 2, 3, l // 3, l // 2, (2 * l) // 3, l - 1, l})
+        dls[f"big{l}_rowidx"] = np.array(rows, dtype=np.int64)
+        dls[f"big{l}_rows"] = np.ascontiguousarray(Q[rows, :])
+        idx = rng.integers(0, l + 1, size=(3000, 2))
+        dls[f"big{l}_sidx"] = idx.astype(np.int64)
+        dls[f"big{l}_svals"] = Q[idx[:, 0], idx[:, 1]]
+        dls[f"big{l}_neg"] = np.ascontiguousarray(P[l:, :l][:, ::-1][:, :8])  # d^l_{m', -k}, k=1..8
+    np.savez_compressed(os.path.join(HERE, "ref_dl.npz"), **dls)
+    # --- a Wigner plane at a generic beta (the recursion's general branch)
+    gen = {}
+    for l, beta in ((5, 0.3), (12, 1.1), (20, 2.5)):
+        b = run("dlbeta", l, beta)
+        gen[f"l{l}_b{beta}"] = np.frombuffer(b, dtype="<f8").reshape((2 * l + 1, 2 * l + 1), order="F")
+    np.savez_compressed(os.path.join(HERE, "ref_dl_beta.npz"), **gen)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
