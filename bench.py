@@ -1,0 +1,216 @@
+"""Benchmark of the S2DW directional wavelet transform on one MI355X.
+
+Workload: L=512, N=8, alpha=2, J_min=2 (reference J = J_max - 2 = 7),
+seeded real-signal f_lm ~ N(0,1) scaled by (l+1)^-1.  One step = one analysis
+plus one synthesis (two transforms) on a device-resident f_lm; inputs are in
+HBM before the timed region.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|split]
+                    [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "transforms/s at L=512,N=8 (split-prec vs FP64, err≤1e-10); p50/p99 latency"
+L, N, ALPHA, J_MIN = 512, 8, 2.0, 2
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def seeded_flm(B, seed, red=True):
+    rng = np.random.default_rng(seed)
+    f = np.zeros((B, B), dtype=np.complex128, order="F")
+    for l in range(B):
+        f[l, 0] = rng.standard_normal()
+        if l:
+            f[l, 1:l + 1] = rng.standard_normal(l) + 1j * rng.standard_normal(l)
+        if red:
+            f[l, :] /= (l + 1)
+    return f
+
+
+def contraction_cmacs(hi, lo, N):
+    """Useful complex MACs of the l-contraction, per transform:
+    analysis sum_j sum_{l=lo_j+1}^{hi_j-1} n_a(l) (l+1)^2 with n_a(l) = #{n: |n| <= min(N-1,l), N+n odd};
+    synthesis the same with n >= 0 only."""
+    ana = syn = 0
+    for h, lo_ in zip(hi, lo):
+        for l in range(lo_ + 1, h):
+            nmax = min(N - 1, l)
+            na = sum(1 for n in range(-nmax, nmax + 1) if (N + n) % 2)
+            ns = sum(1 for n in range(0, nmax + 1) if (N + n) % 2)
+            ana += na * (l + 1) ** 2
+            syn += ns * (l + 1) ** 2
+    return ana, syn
+
+
+def stage_bytes(hi, lo, N, ncs):
+    """Compulsory HBM bytes per stage launch group (all scales) for one transform."""
+    b = {}
+    sc = [(h, l_) for h, l_ in zip(hi, lo) if l_ + 1 <= h - 1]
+    b["ana_beta_fft"] = sum(N * h * h * 16 + N * 2 * h * h * 16 for h, _ in sc)
+    b["ana_alpha_c2r"] = sum(N * 2 * h * h * 16 + N * 2 * h * (2 * h - 1) * 8 for h, _ in sc)
+    b["syn_alpha_r2c"] = sum(N * 2 * h * (2 * h - 1) * 8 + N * 2 * h * h * 16 for h, _ in sc)
+    b["syn_beta_fft"] = sum(N * 2 * h * h * 16 + ncs * h * h * 16 for h, _ in sc)
+    return b
+
+
+def cpu_baseline(K, Phi2, S, J, bl, flm):
+    """The CPU oracle (a C/OpenMP restatement of the reference's algorithm) on the
+    host's cores: one analysis + one synthesis of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    cores = min(16, os.cpu_count() or 1)
+    O.set_threads(cores)
+    t0 = time.perf_counter()
+    w, sc = O.analysis_dynamic(flm, K, S, J, L, N, bl, ALPHA)
+    t1 = time.perf_counter()
+    g = O.synthesis_dynamic(w, sc, K, Phi2, S, J, L, N, bl, ALPHA)
+    t2 = time.perf_counter()
+    err = float(np.abs(g - flm).max())
+    return {"value": 2.0 / (t2 - t0), "unit": "transforms/s", "cores": cores, "kind": "port",
+            "sample": f"one analysis ({t1 - t0:.1f} s) + one synthesis ({t2 - t1:.1f} s) at L=512 N=8 J=7 "
+                      f"by oracle/s2dw_oracle.c (reference loop structure, OpenMP, {cores} threads); "
+                      f"round-trip err {err:.1e}",
+            "analysis_s": t1 - t0, "synthesis_s": t2 - t1}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "split"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(requests shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from s2dw_amd import core
+
+    dev = torch.device("cuda:0")
+    J = core.s2dw_core_comp_Jmax(L, ALPHA) - J_MIN
+    K, Phi2, bl = core.s2dw_core_init_kernels(J, L, ALPHA)
+    S = core.s2dw_core_init_directionality(L, N)
+    flm = seeded_flm(L, 1234)
+    prec = core.PREC_FP64 if args.precision == "fp64" else core.PREC_SPLIT
+    stream = torch.cuda.current_stream(dev)
+    plan = core.Plan(L, N, J, ALPHA, K, Phi2, S, precision=prec, stream=stream)
+    d_f = torch.from_numpy(np.ascontiguousarray(flm.T)).to(dev)
+    d_w, d_sc, d_g = plan.empty_wav(dev), plan.empty_scoeff(dev), plan.empty_flm(dev)
+
+    for _ in range(args.warmup):
+        plan.analysis(d_f, d_w, d_sc)
+        plan.synthesis(d_w, d_sc, d_g)
+    torch.cuda.synchronize()
+    plan.stage_times()  # discard anything recorded so far
+
+    steps = args.steps
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    plan.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[i][0].record(stream)
+        plan.analysis(d_f, d_w, d_sc)
+        ev[i][1].record(stream)
+        plan.synthesis(d_w, d_sc, d_g)
+        ev[i][2].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plan.set_timing(False)
+    wall = t1 - t0
+    st = plan.stage_times()
+    lat_a = [ev[i][0].elapsed_time(ev[i][1]) for i in range(steps)]
+    lat_s = [ev[i][1].elapsed_time(ev[i][2]) for i in range(steps)]
+    lat_all = sorted(lat_a + lat_s)
+
+    g = d_g.cpu().numpy().T
+    rt_err = float(np.abs(g - flm).max())
+
+    hi, lo = plan.bl_hi, plan.bl_lo
+    ana_cmac, syn_cmac = contraction_cmacs(hi, lo, N)
+    sb = stage_bytes(hi, lo, N, ncs=sum(1 for n in range(N) if (N + n) % 2))
+    stages = {}
+    for name, (ms, cnt) in st.items():
+        if cnt:
+            stages[name] = {"ms_per_step": ms / steps, "launches_per_step": cnt / steps}
+    # dominant kernel and its roofline
+    dom = max(stages, key=lambda k: stages[k]["ms_per_step"])
+    dom_ms = stages[dom]["ms_per_step"]
+    if dom in ("ana_contract", "syn_contract"):
+        cm = ana_cmac if dom == "ana_contract" else syn_cmac
+        work = 8.0 * cm  # 8 real flops per useful complex MAC
+        achieved = work / (dom_ms * 1e-3) / 1e12
+        roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "work_per_launch": work / max(1, stages[dom]["launches_per_step"]),
+                "work_unit": "flop (8 per useful complex MAC)"}
+    else:
+        byts = sb.get(dom, 0)
+        achieved = byts / (dom_ms * 1e-3) / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "work_per_launch": byts / max(1, stages[dom]["launches_per_step"]), "work_unit": "byte"}
+    contraction = {}
+    for name, cm in (("ana_contract", ana_cmac), ("syn_contract", syn_cmac)):
+        if name in stages:
+            tf = 8.0 * cm / (stages[name]["ms_per_step"] * 1e-3) / 1e12
+            contraction[name] = {"tflops": tf, "frac_fp64_peak": tf / FP64_PEAK_TFLOPS}
+    ffts = {}
+    for name, byts in sb.items():
+        if name in stages:
+            gbs = byts / (stages[name]["ms_per_step"] * 1e-3) / 1e9
+            ffts[name] = {"GBps": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS}
+
+    out = {
+        "metric": METRIC,
+        "value": 2.0 * steps / wall,
+        "unit": "transforms/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64 (contraction split on MFMA)",
+        "data": "synthetic: seeded real-signal flm, Re/Im ~ N(0,1) scaled by (l+1)^-1",
+        "config": {"workload": "analysis+synthesis round trip, L=512 N=8 alpha=2 J_min=2 (J=7)", "L": L, "N": N,
+                   "alpha": ALPHA, "J": J, "bl_scoeff": bl, "precision": args.precision,
+                   "transforms_per_step": 2},
+        "roofline": roof,
+        "latency_ms": {"analysis_p50": float(np.percentile(lat_a, 50)),
+                       "analysis_p99": float(np.percentile(lat_a, 99)),
+                       "synthesis_p50": float(np.percentile(lat_s, 50)),
+                       "synthesis_p99": float(np.percentile(lat_s, 99)),
+                       "p50": float(np.percentile(lat_all, 50)), "p99": float(np.percentile(lat_all, 99))},
+        "round_trip_max_err": rt_err,
+        "stages": stages,
+        "contraction": contraction,
+        "fft_stages": ffts,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(K, Phi2, S, J, bl, flm)
+    plan.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,190 @@
+/*
+ * s2dw_hip.h -- C ABI of libs2dw_hip.so, the MI355X (gfx950) implementation of
+ * the S2DW scale-discretised directional wavelet transform on the sphere.
+ *
+ * Every entry point replaces one public procedure of the reference's Fortran
+ * module s2dw_core_mod (src/mod/s2dw_core_mod.f90, public list at 29-39) and
+ * keeps its argument order, meaning and array layout, so that a Fortran
+ * module with the same procedure names (s2dw_amd/fortran/s2dw_core_mod.f90)
+ * can forward to it through ISO_C_BINDING.
+ *
+ * Conventions (all identical to the reference):
+ *  - B is the harmonic band limit (0 <= l < B), N the azimuthal band limit,
+ *    J the number of scales minus one (scales jj = 0..J), alpha the dilation.
+ *  - Arrays are Fortran column-major, first index fastest.  complex(dpc)
+ *    arrays are interleaved (re, im) doubles:
+ *      flm(0:B-1,0:B-1)        flm[2*(l + B*m) + {0,1}]
+ *      K_gamma(0:J,0:B-1)      K[j + (J+1)*l]
+ *      Phi2(0:B-1)             Phi2[l]
+ *      Slm(0:B-1,0:N-1)        S[2*(l + B*n) + {0,1}]
+ *      scoeff(0:bl-1,0:bl-1)   sc[2*(l + bl*m) + {0,1}]
+ *      wav(0:J,0:2B-2,0:2B-1,0:N-1) (static)   wav[j + (J+1)*(a + (2B-1)*(b + 2B*g))]
+ *      wavdyn(jj)%coeff(0:2hi-2,0:2hi-1,0:N-1) (dynamic, one array per scale)
+ *                              w[a + (2hi-1)*(b + 2hi*g)]
+ *  - Integers are Fortran default integers (int32), alpha is real(dp).
+ *  - Every function returns an S2DW error code (numbering of
+ *    s2dw_error_mod.f90:37-52); 0 means success.  The reference halts inside
+ *    the failing routine instead (s2dw_error, error_mod.f90:127-191); the
+ *    Fortran module re-raises the returned code through s2dw_error, which
+ *    reproduces the message and the halt.
+ *  - The transform entry points take host pointers, copy to the GPU, run and
+ *    copy back before returning (synchronous, like the Fortran routines).
+ *    The *_device entry points take device pointers and a HIP stream and
+ *    return after enqueueing.
+ *  - Thread safety: calls on distinct plans may run concurrently from
+ *    different host threads; calls on one plan must not overlap.
+ */
+#ifndef S2DW_HIP_H
+#define S2DW_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes, s2dw_error_mod.f90:37-52. */
+#define S2DW_ERROR_NONE 0
+#define S2DW_ERROR_INIT 1
+#define S2DW_ERROR_NOT_INIT 2
+#define S2DW_ERROR_INIT_FAIL 3
+#define S2DW_ERROR_MEM_ALLOC_FAIL 4
+#define S2DW_ERROR_ARTH 5
+#define S2DW_ERROR_ARTH_WARNING 6
+#define S2DW_ERROR_SIZE_WARNING 7
+#define S2DW_ERROR_SIZE_INVALID 8
+#define S2DW_ERROR_SIZE_NOT_DEF 9
+#define S2DW_ERROR_ARG_INVALID 10
+#define S2DW_ERROR_ARG_WARNING 11
+#define S2DW_ERROR_QUAD_STEP_EXCEED 12
+#define S2DW_ERROR_ADMISS_FAIL 13
+#define S2DW_ERROR_FILEIO 14
+
+/* Precision of the l-contraction (the rest of the path is always FP64). */
+#define S2DW_PREC_FP64 0  /* plain form: everything in FP64 */
+#define S2DW_PREC_SPLIT 1 /* split-precision contraction on MFMA (see plan docs) */
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *s2dw_hip_last_error(void);
+
+/* Library version string. */
+const char *s2dw_hip_version(void);
+
+/* ---------------------------------------------------------------------
+ * Setup routines (host, FP64).  These run on the CPU: they are once-per-
+ * configuration quadratures whose results are inputs at the boundary.
+ * ------------------------------------------------------------------- */
+
+/* s2dw_core_init_kernels, s2dw_core_mod.f90:97-157.  Writes K_gamma
+ * ((J+1)*B), Phi2 (B) and *bl_scoeff. */
+int s2dw_hip_init_kernels(double *K_gamma, double *Phi2, int *bl_scoeff, int J, int B, double alpha);
+
+/* s2dw_core_init_directionality, s2dw_core_mod.f90:219-250.  Writes Slm
+ * (B*N complex). */
+int s2dw_hip_init_directionality(double *Slm, int B, int N);
+
+/* s2dw_core_comp_Jmax, s2dw_core_mod.f90:1365-1376.  Returns J_max. */
+int s2dw_hip_comp_Jmax(int B, double alpha);
+
+/* s2dw_core_admiss, s2dw_core_mod.f90:285-304.  Writes admiss (B) and
+ * *pass (1 if |admiss-1| <= TOL_ADMISS for all l, else 0). */
+int s2dw_hip_admiss(double *admiss, const double *K_gamma, const double *Phi2, int B, int J, int *pass);
+
+/* s2dw_core_assimilate_int, s2dw_core_mod.f90:1398-1415.  Returns 1 and
+ * sets *y when x is within TOL_CEIL of an integer, else returns 0, *y = 0. */
+int s2dw_hip_assimilate_int(double x, int *y);
+
+/* Band limits (bl_hi, bl_lo) of every scale jj = 0..J, exactly as the
+ * transform routines compute them (s2dw_core_mod.f90:606-616). */
+int s2dw_hip_band_limits(int J, int B, double alpha, int *bl_hi, int *bl_lo);
+
+/* ---------------------------------------------------------------------
+ * Transforms, host pointers, synchronous (drop-in for the Fortran API).
+ * ------------------------------------------------------------------- */
+
+/* s2dw_core_analysis_flm2wav, s2dw_core_mod.f90:342-534 (static wav). */
+int s2dw_hip_analysis_flm2wav(double *wav, double *scoeff, const double *flm, const double *K_gamma,
+                              const double *Slm, int J, int B, int N, int bl_scoeff, double alpha);
+
+/* s2dw_core_analysis_flm2wav_dynamic, s2dw_core_mod.f90:572-775.
+ * wavdyn[jj] points to caller-allocated (2hi-1)*2hi*N doubles for scale jj
+ * (the Fortran module allocates wavdyn(jj)%coeff and passes c_loc). */
+int s2dw_hip_analysis_flm2wav_dynamic(double *const *wavdyn, double *scoeff, const double *flm,
+                                      const double *K_gamma, const double *Slm, int J, int B, int N,
+                                      int bl_scoeff, double alpha);
+
+/* s2dw_core_synthesis_wav2flm, s2dw_core_mod.f90:816-1019 (static wav). */
+int s2dw_hip_synthesis_wav2flm(double *flm, const double *wav, const double *scoeff, const double *K_gamma,
+                               const double *Phi2, const double *Slm, int J, int B, int N, int bl_scoeff,
+                               double alpha);
+
+/* s2dw_core_synthesis_wav2flm_dynamic, s2dw_core_mod.f90:1060-1266.  Reads
+ * wavdyn[jj]; freeing the Fortran allocations (1130, 1264) is left to the
+ * Fortran module. */
+int s2dw_hip_synthesis_wav2flm_dynamic(double *flm, const double *const *wavdyn, const double *scoeff,
+                                       const double *K_gamma, const double *Phi2, const double *Slm, int J,
+                                       int B, int N, int bl_scoeff, double alpha);
+
+/* ---------------------------------------------------------------------
+ * Plans: device-resident transforms.  A plan holds the per-configuration
+ * device tables (Wigner chain starts and the thin Delta^l_{m'n} table, FFT
+ * twiddles, kernel and directionality weights) and scratch, and runs on
+ * one HIP stream.  The reference recomputes these per call; the plan
+ * computes them once.
+ * ------------------------------------------------------------------- */
+typedef struct s2dw_hip_plan *s2dw_hip_plan_t;
+
+/* Create a plan for (B, N, J, alpha) with the caller's K_gamma, Phi2, Slm
+ * (layouts as above) and the chosen contraction precision.  All work of the
+ * plan is enqueued on `stream` (a hipStream_t; NULL is the null stream). */
+int s2dw_hip_plan_create(s2dw_hip_plan_t *plan, int B, int N, int J, double alpha, const double *K_gamma,
+                         const double *Phi2, const double *Slm, int precision, void *stream);
+int s2dw_hip_plan_destroy(s2dw_hip_plan_t plan);
+
+/* Sizes: number of doubles of the packed wav (all scales, dynamic layout
+ * concatenated in jj order), bl_scoeff, and per-scale offsets (J+1 entries,
+ * in doubles) and band limits.  Any pointer may be NULL. */
+int s2dw_hip_plan_sizes(s2dw_hip_plan_t plan, long long *wav_doubles, int *bl_scoeff, long long *wav_offsets,
+                        int *bl_hi, int *bl_lo);
+
+/* Device-memory bytes the plan holds (tables + scratch). */
+long long s2dw_hip_plan_device_bytes(s2dw_hip_plan_t plan);
+
+/* Analysis on device pointers: d_flm (B*B complex), d_wav (packed per-scale
+ * dynamic layout, see plan_sizes), d_scoeff (bl*bl complex).  Enqueued on
+ * the plan's stream; returns without synchronising. */
+int s2dw_hip_analysis_device(s2dw_hip_plan_t plan, double *d_wav, double *d_scoeff, const double *d_flm);
+
+/* Synthesis on device pointers (layouts as analysis_device). */
+int s2dw_hip_synthesis_device(s2dw_hip_plan_t plan, double *d_flm, const double *d_wav, const double *d_scoeff);
+
+/* Wait for the plan's stream. */
+int s2dw_hip_plan_sync(s2dw_hip_plan_t plan);
+
+/* The plan's HIP stream (hipStream_t as void*). */
+void *s2dw_hip_plan_stream(s2dw_hip_plan_t plan);
+
+/* Per-stage timing with HIP events on the plan's stream.  Stage ids:
+ * 0 analysis f_lm relayout, 1 analysis l-contraction (incl. gamma-DFT),
+ * 2 analysis beta FFTs, 3 analysis alpha c2r FFTs, 4 scaling coefficients,
+ * 5 synthesis alpha r2c FFTs, 6 synthesis gamma + beta quadrature FFTs,
+ * 7 synthesis Wigner contraction, 8 synthesis f_lm assembly.
+ * No counterpart in the reference (which prints wall-clock only,
+ * s2dw_test.f90:97-135). */
+#define S2DW_NUM_STAGES 9
+int s2dw_hip_plan_set_timing(s2dw_hip_plan_t plan, int enable);
+/* Sums (ms) and launch counts per stage of everything recorded since the
+ * last call; waits for the recorded work.  ms and count hold 9 entries. */
+int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count);
+
+/* ---------------------------------------------------------------------
+ * Diagnostics used by the tests: Wigner values Delta^l_{m'm}(pi/2) for
+ * 0 <= m', m <= l generated by the device recursions of the transform
+ * kernels.  out[m' + (l+1)*m] for the analysis (l-recursion) generator,
+ * out2 (may be NULL) for the synthesis (m'-recursion) generator.
+ * ------------------------------------------------------------------- */
+int s2dw_hip_wigner_plane(int l, double *out, double *out2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* S2DW_HIP_H */

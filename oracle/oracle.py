@@ -61,6 +61,7 @@ def lib():
                                            C.c_int, C.c_double]
         L.orc_test_gen_flm.argtypes = [C.c_int, _dp, C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_dl_precise.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -71,6 +72,11 @@ def _p(a: np.ndarray):
 
 def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
+
+
+def set_dl_precise(on: bool) -> None:
+    """Diagnostic: compute Wigner planes with the same recursion in long double."""
+    lib().orc_set_dl_precise(int(bool(on)))
 
 
 def get_threads() -> int:

@@ -292,87 +292,88 @@ double orc_quad_weights(double theta_t, int B) {
  * ---------------------------------------------------------------------- */
 #define DLI(i, j) (((i)-1) + (size_t)(2 * l + 1) * ((j)-1)) /* 1-based (i,j) */
 
-static void orc_dl_core(double *dl, double beta, int l) {
-  const int n = 2 * l + 1;
-  if (fabs(beta) < 1e-5) {
-    for (size_t k = 0; k < (size_t)n * n; k++) dl[k] = 0.0;
-    for (int i = 1; i <= n; i++) dl[DLI(i, i)] = 1.0;
-    return;
+/* The operator recursion of s2dw_dl_beta_operator_core (s2dw_dl_mod.f90:
+ * 92-238), written once for a floating type R (double: the restatement;
+ * long double: the extended-precision diagnostic).  Output rounded to double. */
+#define ORC_DL_CORE(NAME, R, COS, SIN, TAN, LOG, EXP, SQRT, FABS)                                   \
+  static void NAME(double *out, double beta_in, int l) {                                           \
+    const int n = 2 * l + 1;                                                                       \
+    R *dl = (R *)calloc((size_t)n * n, sizeof(R));                                                 \
+    const R beta = (R)beta_in;                                                                     \
+    if (FABS(beta) < (R)1e-5) {                                                                    \
+      for (int i = 1; i <= n; i++) dl[DLI(i, i)] = 1;                                              \
+    } else {                                                                                       \
+      const R big_const = (R)1.0e150;                                                              \
+      R *cp = (R *)calloc(n + 1, sizeof(R)), *cpi = (R *)calloc(n + 1, sizeof(R));                 \
+      R *cp2 = (R *)calloc(n + 1, sizeof(R)), *lfr = (R *)calloc(n + 1, sizeof(R));                \
+      R *sgn = (R *)calloc(n + 1, sizeof(R)), *lrenorm = (R *)calloc(n + 1, sizeof(R));            \
+      R ang = beta;                                                                                \
+      R c = COS(ang), s = SIN(ang), si = 1 / s, t = TAN(-ang / 2), c2 = COS(ang / 2);              \
+      R omc = 1 - c;                                                                               \
+      lfr[1] = (2 * (R)l) * LOG(FABS(c2));                                                         \
+      sgn[1] = 1;                                                                                  \
+      for (int i = 2; i <= n; i++) {                                                               \
+        int m = l + 1 - i;                                                                         \
+        R ratio = SQRT((R)(l + m + 1) / (R)(l - m));                                               \
+        lfr[i] = lfr[i - 1] + LOG(ratio) + LOG(FABS(t));                                           \
+        sgn[i] = sgn[i - 1] * t / FABS(t);                                                         \
+      }                                                                                            \
+      R big = big_const, lbig = LOG(big), bigi = 1 / big_const;                                    \
+      R xllp1 = (R)(l * (l + 1));                                                                  \
+      for (int m = 1; m <= l; m++) {                                                               \
+        R xm = (R)(l - m);                                                                         \
+        cpi[m] = 2 / SQRT(xllp1 - xm * (xm + 1));                                                  \
+        cp[m] = 1 / cpi[m];                                                                        \
+      }                                                                                            \
+      for (int m = 2; m <= l; m++) cp2[m] = cpi[m] * cp[m - 1];                                    \
+      dl[DLI(1, 1)] = 1;                                                                           \
+      dl[DLI(2 * l + 1, 1)] = 1;                                                                   \
+      for (int index = 2; index <= l + 1; index++) {                                               \
+        dl[DLI(index, 1)] = 1;                                                                     \
+        R lambda = ((R)(l + 1) * omc - (R)index + c) * si;                                         \
+        dl[DLI(index, 2)] = lambda * dl[DLI(index, 1)] * cpi[1];                                   \
+        if (index > 2) {                                                                           \
+          for (int m = 2; m <= index - 1; m++) {                                                   \
+            lambda = ((R)(l + 1) * omc - (R)index + (R)m * c) / s;                                 \
+            dl[DLI(index, m + 1)] = lambda * cpi[m] * dl[DLI(index, m)] - cp2[m] * dl[DLI(index, m - 1)]; \
+            if (dl[DLI(index, m + 1)] > big) {                                                     \
+              lrenorm[index] = lrenorm[index] - lbig;                                              \
+              for (int im = 1; im <= m + 1; im++) dl[DLI(index, im)] = dl[DLI(index, im)] * bigi;  \
+            }                                                                                      \
+          }                                                                                        \
+        }                                                                                          \
+      }                                                                                            \
+      for (int index = l + 2; index <= 2 * l; index++) {                                           \
+        dl[DLI(index, 1)] = 1;                                                                     \
+        R lambda = ((R)(l + 1) * omc - (R)index + c) / s;                                          \
+        dl[DLI(index, 2)] = lambda * dl[DLI(index, 1)] * cpi[1];                                   \
+        if (index < 2 * l) {                                                                       \
+          for (int m = 2; m <= 2 * l - index + 1; m++) {                                           \
+            lambda = ((R)(l + 1) * omc - (R)index + (R)m * c) / s;                                 \
+            dl[DLI(index, m + 1)] = lambda * cpi[m] * dl[DLI(index, m)] - cp2[m] * dl[DLI(index, m - 1)]; \
+            if (dl[DLI(index, m + 1)] > big) {                                                     \
+              for (int im = 1; im <= m + 1; im++) dl[DLI(index, im)] = dl[DLI(index, im)] * bigi;  \
+              lrenorm[index] = lrenorm[index] - lbig;                                              \
+            }                                                                                      \
+          }                                                                                        \
+        }                                                                                          \
+      }                                                                                            \
+      for (int i = 1; i <= l + 1; i++) {                                                           \
+        R renorm = sgn[i] * EXP(lfr[i] - lrenorm[i]);                                              \
+        for (int j = 1; j <= i; j++) dl[DLI(i, j)] = dl[DLI(i, j)] * renorm;                       \
+      }                                                                                            \
+      for (int i = l + 2; i <= 2 * l + 1; i++) {                                                   \
+        R renorm = sgn[i] * EXP(lfr[i] - lrenorm[i]);                                              \
+        for (int j = 1; j <= 2 * l + 2 - i; j++) dl[DLI(i, j)] = dl[DLI(i, j)] * renorm;           \
+      }                                                                                            \
+      free(cp); free(cpi); free(cp2); free(lfr); free(sgn); free(lrenorm);                        \
+    }                                                                                              \
+    for (size_t k = 0; k < (size_t)n * n; k++) out[k] = (double)dl[k];                             \
+    free(dl);                                                                                      \
   }
-  const double big_const = 1.0e150;
-  double *cp = (double *)calloc(n + 1, sizeof(double));
-  double *cpi = (double *)calloc(n + 1, sizeof(double));
-  double *cp2 = (double *)calloc(n + 1, sizeof(double));
-  double *lfr = (double *)calloc(n + 1, sizeof(double));
-  double *sgn = (double *)calloc(n + 1, sizeof(double));
-  double *lrenorm = (double *)calloc(n + 1, sizeof(double));
-  double ang = beta;
-  double c = cos(ang), s = sin(ang), si = 1.0 / s, t = tan(-ang / 2.0), c2 = cos(ang / 2.0);
-  double omc = 1.0 - c;
-  for (int i = 1; i <= n; i++) lrenorm[i] = 0.0;
-  lfr[1] = (2.0 * (double)l) * log(fabs(c2));
-  sgn[1] = 1.0;
-  for (int i = 2; i <= n; i++) {
-    int m = l + 1 - i;
-    double ratio = sqrt((double)(l + m + 1) / (double)(l - m));
-    lfr[i] = lfr[i - 1] + log(ratio) + log(fabs(t));
-    sgn[i] = sgn[i - 1] * t / fabs(t);
-  }
-  double big = big_const, lbig = log(big), bigi = 1.0 / big_const;
-  double xllp1 = (double)(l * (l + 1));
-  for (int m = 1; m <= l; m++) {
-    double xm = (double)(l - m);
-    cpi[m] = 2.0 / sqrt(xllp1 - xm * (xm + 1));
-    cp[m] = 1.0 / cpi[m];
-  }
-  for (int m = 2; m <= l; m++) cp2[m] = cpi[m] * cp[m - 1];
-  dl[DLI(1, 1)] = 1.0;
-  dl[DLI(2 * l + 1, 1)] = 1.0;
-  for (int index = 2; index <= l + 1; index++) {
-    dl[DLI(index, 1)] = 1.0;
-    double lambda = ((double)(l + 1) * omc - (double)index + c) * si;
-    dl[DLI(index, 2)] = lambda * dl[DLI(index, 1)] * cpi[1];
-    if (index > 2) {
-      for (int m = 2; m <= index - 1; m++) {
-        lambda = ((double)(l + 1) * omc - (double)index + (double)m * c) / s;
-        dl[DLI(index, m + 1)] = lambda * cpi[m] * dl[DLI(index, m)] - cp2[m] * dl[DLI(index, m - 1)];
-        if (dl[DLI(index, m + 1)] > big) {
-          lrenorm[index] = lrenorm[index] - lbig;
-          for (int im = 1; im <= m + 1; im++) dl[DLI(index, im)] = dl[DLI(index, im)] * bigi;
-        }
-      }
-    }
-  }
-  for (int index = l + 2; index <= 2 * l; index++) {
-    dl[DLI(index, 1)] = 1.0;
-    double lambda = ((double)(l + 1) * omc - (double)index + c) / s;
-    dl[DLI(index, 2)] = lambda * dl[DLI(index, 1)] * cpi[1];
-    if (index < 2 * l) {
-      for (int m = 2; m <= 2 * l - index + 1; m++) {
-        lambda = ((double)(l + 1) * omc - (double)index + (double)m * c) / s;
-        dl[DLI(index, m + 1)] = lambda * cpi[m] * dl[DLI(index, m)] - cp2[m] * dl[DLI(index, m - 1)];
-        if (dl[DLI(index, m + 1)] > big) {
-          for (int im = 1; im <= m + 1; im++) dl[DLI(index, im)] = dl[DLI(index, im)] * bigi;
-          lrenorm[index] = lrenorm[index] - lbig;
-        }
-      }
-    }
-  }
-  for (int i = 1; i <= l + 1; i++) {
-    double renorm = sgn[i] * exp(lfr[i] - lrenorm[i]);
-    for (int j = 1; j <= i; j++) dl[DLI(i, j)] = dl[DLI(i, j)] * renorm;
-  }
-  for (int i = l + 2; i <= 2 * l + 1; i++) {
-    double renorm = sgn[i] * exp(lfr[i] - lrenorm[i]);
-    for (int j = 1; j <= 2 * l + 2 - i; j++) dl[DLI(i, j)] = dl[DLI(i, j)] * renorm;
-  }
-  free(cp);
-  free(cpi);
-  free(cp2);
-  free(lfr);
-  free(sgn);
-  free(lrenorm);
-}
+
+ORC_DL_CORE(orc_dl_core, double, cos, sin, tan, log, exp, sqrt, fabs)
+ORC_DL_CORE(orc_dl_core_ld, long double, cosl, sinl, tanl, logl, expl, sqrtl, fabsl)
 
 static void orc_dl_fill(double *dl, int l) {
   for (int i = 1; i <= l; i++)
@@ -398,8 +399,19 @@ static void orc_dl_fill(double *dl, int l) {
 #undef DLI
 
 /* s2dw_dl_beta_operator, s2dw_dl_mod.f90:56-68 */
+static int orc_dl_precise = 0;
+
+/* Diagnostic switch: 1 computes the Wigner planes with the same recursion in
+ * long double (x87 80-bit) arithmetic, rounded to double at the end.  It shows
+ * how much of a difference against another implementation comes from the
+ * rounding of the reference's double-precision planes.  Default 0. */
+void orc_set_dl_precise(int on) { orc_dl_precise = on; }
+
 void orc_dl_beta_operator(double *dl, double beta, int l) {
-  orc_dl_core(dl, beta, l);
+  if (orc_dl_precise)
+    orc_dl_core_ld(dl, beta, l);
+  else
+    orc_dl_core(dl, beta, l);
   orc_dl_fill(dl, l);
 }
 

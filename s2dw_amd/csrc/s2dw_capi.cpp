@@ -1,0 +1,333 @@
+// s2dw_capi.cpp -- the C ABI of libs2dw_hip.so (declared in include/s2dw_hip.h).
+//
+// Host-pointer entry points mirror the reference's s2dw_core_mod procedures
+// (synchronous, same arguments and layouts).  They run on a cached plan per
+// (device, B, N, J, alpha, K_gamma, Phi2, Slm): the plan's device tables are
+// built once, later calls with the same configuration only move data.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+
+#include "../../include/s2dw_hip.h"
+#include "s2dw_internal.h"
+
+struct s2dw_hip_plan {
+  s2dw::Plan p;
+};
+
+namespace s2dw {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_error(hipError_t e, const char *what) {
+  set_error("HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+  return e == hipErrorOutOfMemory ? kErrMemAlloc : kErrInitFail;
+}
+
+namespace {
+
+// Cached plans and staging buffers for the host-pointer entry points.
+struct HostEntry {
+  Plan plan;
+  double2 *d_flm = nullptr, *d_sc = nullptr;
+  double *d_wav = nullptr, *d_static = nullptr;
+  size_t static_doubles = 0;
+  ~HostEntry() {
+    if (d_flm) (void)hipFree(d_flm);
+    if (d_sc) (void)hipFree(d_sc);
+    if (d_wav) (void)hipFree(d_wav);
+    if (d_static) (void)hipFree(d_static);
+    plan_free(&plan);
+  }
+};
+
+std::mutex g_mu;
+std::list<std::unique_ptr<HostEntry>> g_cache;  // most recent first
+constexpr size_t kCacheMax = 4;
+
+bool same_config(const Plan &p, int dev, int B, int N, int J, double alpha, const double *K, const double *Phi2,
+                 const double *S) {
+  if (p.device != dev || p.B != B || p.N != N || p.J != J || p.alpha != alpha) return false;
+  if (std::memcmp(p.K.data(), K, sizeof(double) * (size_t)(J + 1) * B)) return false;
+  if (Phi2 && std::memcmp(p.Phi2.data(), Phi2, sizeof(double) * (size_t)B)) return false;
+  if (std::memcmp(p.S.data(), S, sizeof(double) * 2 * (size_t)B * N)) return false;
+  return true;
+}
+
+// Returns a plan for the configuration; Phi2 may be NULL for analysis (it is
+// not used there), in which case a matching cached plan of any Phi2 is taken.
+int get_entry(int B, int N, int J, double alpha, const double *K, const double *Phi2, const double *S,
+              HostEntry **out) {
+  int dev = 0;
+  S2DW_HIP_CHECK(hipGetDevice(&dev));
+  for (auto it = g_cache.begin(); it != g_cache.end(); ++it) {
+    if (same_config((*it)->plan, dev, B, N, J, alpha, K, Phi2, S)) {
+      g_cache.splice(g_cache.begin(), g_cache, it);
+      *out = g_cache.front().get();
+      return kErrNone;
+    }
+  }
+  auto e = std::make_unique<HostEntry>();
+  std::vector<double> phi(B, 0.0);
+  if (Phi2) std::memcpy(phi.data(), Phi2, sizeof(double) * B);
+  int err = plan_build(&e->plan, B, N, J, alpha, K, phi.data(), S, S2DW_PREC_FP64, nullptr, true);
+  if (err) return err;
+  const Plan &p = e->plan;
+  S2DW_HIP_CHECK(hipMalloc(&e->d_flm, sizeof(double2) * (size_t)B * B));
+  S2DW_HIP_CHECK(hipMalloc(&e->d_sc, sizeof(double2) * (size_t)p.bl_scoeff * p.bl_scoeff));
+  S2DW_HIP_CHECK(hipMalloc(&e->d_wav, sizeof(double) * (size_t)p.wav_doubles));
+  g_cache.push_front(std::move(e));
+  while (g_cache.size() > kCacheMax) g_cache.pop_back();
+  *out = g_cache.front().get();
+  return kErrNone;
+}
+
+int ensure_static(HostEntry *e) {
+  const Plan &p = e->plan;
+  const size_t n = (size_t)(p.J + 1) * (2 * p.B - 1) * (2 * p.B) * p.N;
+  if (e->static_doubles < n) {
+    if (e->d_static) (void)hipFree(e->d_static);
+    e->d_static = nullptr;
+    S2DW_HIP_CHECK(hipMalloc(&e->d_static, sizeof(double) * n));
+    e->static_doubles = n;
+  }
+  return kErrNone;
+}
+
+int check_bl(const Plan &p, int bl_scoeff) {
+  if (bl_scoeff != p.bl_scoeff) {
+    set_error("bl_scoeff=%d does not match the kernels' band limit %d", bl_scoeff, p.bl_scoeff);
+    return kErrSizeInvalid;
+  }
+  return kErrNone;
+}
+
+}  // namespace
+}  // namespace s2dw
+
+using namespace s2dw;
+
+extern "C" {
+
+const char *s2dw_hip_last_error(void) { return g_err; }
+
+const char *s2dw_hip_version(void) { return "s2dw_hip 0.1 (gfx950)"; }
+
+int s2dw_hip_init_kernels(double *K_gamma, double *Phi2, int *bl_scoeff, int J, int B, double alpha) {
+  if (!K_gamma || !Phi2 || !bl_scoeff) return kErrArgInvalid;
+  const int e = init_kernels(K_gamma, Phi2, bl_scoeff, J, B, alpha);
+  if (e == kErrSizeInvalid) set_error("s2dw_core_init_kernels: invalid sizes J=%d B=%d alpha=%g", J, B, alpha);
+  if (e == kErrQuadStepExceed) set_error("qsimp: exceeded number of steps");
+  return e;
+}
+
+int s2dw_hip_init_directionality(double *Slm, int B, int N) {
+  if (!Slm) return kErrArgInvalid;
+  const int e = init_directionality(Slm, B, N);
+  if (e) set_error("s2dw_core_init_directionality: N=%d > B=%d", N, B);
+  return e;
+}
+
+int s2dw_hip_comp_Jmax(int B, double alpha) { return comp_Jmax(B, alpha); }
+
+int s2dw_hip_admiss(double *a, const double *K_gamma, const double *Phi2, int B, int J, int *pass) {
+  if (!a || !K_gamma || !Phi2) return kErrArgInvalid;
+  const bool ok = admiss(a, K_gamma, Phi2, B, J);
+  if (pass) *pass = ok ? 1 : 0;
+  return kErrNone;
+}
+
+int s2dw_hip_assimilate_int(double x, int *y) {
+  int v = 0;
+  const bool ok = assimilate_int(x, &v);
+  if (y) *y = v;
+  return ok ? 1 : 0;
+}
+
+int s2dw_hip_band_limits(int J, int B, double alpha, int *bl_hi, int *bl_lo) {
+  for (int j = 0; j <= J; ++j) {
+    int h, l;
+    band_limits(j, B, alpha, &h, &l);
+    if (bl_hi) bl_hi[j] = h;
+    if (bl_lo) bl_lo[j] = l;
+  }
+  return kErrNone;
+}
+
+static int analysis_common(double *const *wavdyn, double *wav_static, double *scoeff, const double *flm,
+                           const double *K_gamma, const double *Slm, int J, int B, int N, int bl_scoeff,
+                           double alpha) {
+  if (!scoeff || !flm || !K_gamma || !Slm) return kErrArgInvalid;
+  std::lock_guard<std::mutex> lk(g_mu);
+  HostEntry *e = nullptr;
+  int err = get_entry(B, N, J, alpha, K_gamma, nullptr, Slm, &e);
+  if (err) return err;
+  Plan &p = e->plan;
+  if ((err = check_bl(p, bl_scoeff))) return err;
+  S2DW_HIP_CHECK(hipMemcpyAsync(e->d_flm, flm, sizeof(double2) * (size_t)B * B, hipMemcpyHostToDevice, p.stream));
+  if ((err = run_analysis(&p, e->d_wav, e->d_sc, e->d_flm))) return err;
+  if (wav_static) {
+    if ((err = ensure_static(e))) return err;
+    if ((err = pack_static(&p, e->d_wav, e->d_static, true))) return err;
+    const size_t n = (size_t)(J + 1) * (2 * B - 1) * (2 * B) * N;
+    S2DW_HIP_CHECK(hipMemcpyAsync(wav_static, e->d_static, sizeof(double) * n, hipMemcpyDeviceToHost, p.stream));
+  } else {
+    for (int j = 0; j <= J; ++j) {
+      const Scale &s = p.sc[j];
+      const size_t n = (size_t)(2 * s.hi - 1) * 2 * s.hi * N;
+      S2DW_HIP_CHECK(
+          hipMemcpyAsync(wavdyn[j], e->d_wav + s.wav_off, sizeof(double) * n, hipMemcpyDeviceToHost, p.stream));
+    }
+  }
+  S2DW_HIP_CHECK(hipMemcpyAsync(scoeff, e->d_sc, sizeof(double2) * (size_t)bl_scoeff * bl_scoeff,
+                                hipMemcpyDeviceToHost, p.stream));
+  S2DW_HIP_CHECK(hipStreamSynchronize(p.stream));
+  return kErrNone;
+}
+
+int s2dw_hip_analysis_flm2wav(double *wav, double *scoeff, const double *flm, const double *K_gamma,
+                              const double *Slm, int J, int B, int N, int bl_scoeff, double alpha) {
+  if (!wav) return kErrArgInvalid;
+  return analysis_common(nullptr, wav, scoeff, flm, K_gamma, Slm, J, B, N, bl_scoeff, alpha);
+}
+
+int s2dw_hip_analysis_flm2wav_dynamic(double *const *wavdyn, double *scoeff, const double *flm,
+                                      const double *K_gamma, const double *Slm, int J, int B, int N,
+                                      int bl_scoeff, double alpha) {
+  if (!wavdyn) return kErrArgInvalid;
+  for (int j = 0; j <= J; ++j)
+    if (!wavdyn[j]) return kErrArgInvalid;
+  return analysis_common(wavdyn, nullptr, scoeff, flm, K_gamma, Slm, J, B, N, bl_scoeff, alpha);
+}
+
+static int synthesis_common(double *flm, const double *const *wavdyn, const double *wav_static, const double *scoeff,
+                            const double *K_gamma, const double *Phi2, const double *Slm, int J, int B, int N,
+                            int bl_scoeff, double alpha) {
+  if (!flm || !scoeff || !K_gamma || !Phi2 || !Slm) return kErrArgInvalid;
+  std::lock_guard<std::mutex> lk(g_mu);
+  HostEntry *e = nullptr;
+  int err = get_entry(B, N, J, alpha, K_gamma, Phi2, Slm, &e);
+  if (err) return err;
+  Plan &p = e->plan;
+  if ((err = check_bl(p, bl_scoeff))) return err;
+  if (wav_static) {
+    if ((err = ensure_static(e))) return err;
+    const size_t n = (size_t)(J + 1) * (2 * B - 1) * (2 * B) * N;
+    S2DW_HIP_CHECK(hipMemcpyAsync(e->d_static, wav_static, sizeof(double) * n, hipMemcpyHostToDevice, p.stream));
+    if ((err = pack_static(&p, e->d_wav, e->d_static, false))) return err;
+  } else {
+    for (int j = 0; j <= J; ++j) {
+      const Scale &s = p.sc[j];
+      const size_t n = (size_t)(2 * s.hi - 1) * 2 * s.hi * N;
+      S2DW_HIP_CHECK(
+          hipMemcpyAsync(e->d_wav + s.wav_off, wavdyn[j], sizeof(double) * n, hipMemcpyHostToDevice, p.stream));
+    }
+  }
+  S2DW_HIP_CHECK(hipMemcpyAsync(e->d_sc, scoeff, sizeof(double2) * (size_t)bl_scoeff * bl_scoeff,
+                                hipMemcpyHostToDevice, p.stream));
+  if ((err = run_synthesis(&p, e->d_flm, e->d_wav, e->d_sc))) return err;
+  S2DW_HIP_CHECK(hipMemcpyAsync(flm, e->d_flm, sizeof(double2) * (size_t)B * B, hipMemcpyDeviceToHost, p.stream));
+  S2DW_HIP_CHECK(hipStreamSynchronize(p.stream));
+  return kErrNone;
+}
+
+int s2dw_hip_synthesis_wav2flm(double *flm, const double *wav, const double *scoeff, const double *K_gamma,
+                               const double *Phi2, const double *Slm, int J, int B, int N, int bl_scoeff,
+                               double alpha) {
+  if (!wav) return kErrArgInvalid;
+  return synthesis_common(flm, nullptr, wav, scoeff, K_gamma, Phi2, Slm, J, B, N, bl_scoeff, alpha);
+}
+
+int s2dw_hip_synthesis_wav2flm_dynamic(double *flm, const double *const *wavdyn, const double *scoeff,
+                                       const double *K_gamma, const double *Phi2, const double *Slm, int J,
+                                       int B, int N, int bl_scoeff, double alpha) {
+  if (!wavdyn) return kErrArgInvalid;
+  for (int j = 0; j <= J; ++j)
+    if (!wavdyn[j]) return kErrArgInvalid;
+  return synthesis_common(flm, wavdyn, nullptr, scoeff, K_gamma, Phi2, Slm, J, B, N, bl_scoeff, alpha);
+}
+
+int s2dw_hip_plan_create(s2dw_hip_plan_t *plan, int B, int N, int J, double alpha, const double *K_gamma,
+                         const double *Phi2, const double *Slm, int precision, void *stream) {
+  if (!plan || !K_gamma || !Phi2 || !Slm) return kErrArgInvalid;
+  if (precision != S2DW_PREC_FP64 && precision != S2DW_PREC_SPLIT) return kErrArgInvalid;
+  auto *h = new (std::nothrow) s2dw_hip_plan;
+  if (!h) return kErrMemAlloc;
+  const int e = plan_build(&h->p, B, N, J, alpha, K_gamma, Phi2, Slm, precision, (hipStream_t)stream, false);
+  if (e) {
+    plan_free(&h->p);
+    delete h;
+    *plan = nullptr;
+    return e;
+  }
+  *plan = h;
+  return kErrNone;
+}
+
+int s2dw_hip_plan_destroy(s2dw_hip_plan_t plan) {
+  if (!plan) return kErrArgInvalid;
+  plan_free(&plan->p);
+  delete plan;
+  return kErrNone;
+}
+
+int s2dw_hip_plan_sizes(s2dw_hip_plan_t plan, long long *wav_doubles, int *bl_scoeff, long long *wav_offsets,
+                        int *bl_hi, int *bl_lo) {
+  if (!plan) return kErrArgInvalid;
+  const Plan &p = plan->p;
+  if (wav_doubles) *wav_doubles = p.wav_doubles;
+  if (bl_scoeff) *bl_scoeff = p.bl_scoeff;
+  for (int j = 0; j <= p.J; ++j) {
+    if (wav_offsets) wav_offsets[j] = p.sc[j].wav_off;
+    if (bl_hi) bl_hi[j] = p.sc[j].hi;
+    if (bl_lo) bl_lo[j] = p.sc[j].lo;
+  }
+  return kErrNone;
+}
+
+long long s2dw_hip_plan_device_bytes(s2dw_hip_plan_t plan) { return plan ? plan->p.device_bytes : -1; }
+
+int s2dw_hip_analysis_device(s2dw_hip_plan_t plan, double *d_wav, double *d_scoeff, const double *d_flm) {
+  if (!plan || !d_wav || !d_scoeff || !d_flm) return kErrArgInvalid;
+  return run_analysis(&plan->p, d_wav, reinterpret_cast<double2 *>(d_scoeff), reinterpret_cast<const double2 *>(d_flm));
+}
+
+int s2dw_hip_synthesis_device(s2dw_hip_plan_t plan, double *d_flm, const double *d_wav, const double *d_scoeff) {
+  if (!plan || !d_wav || !d_scoeff || !d_flm) return kErrArgInvalid;
+  return run_synthesis(&plan->p, reinterpret_cast<double2 *>(d_flm), d_wav, reinterpret_cast<const double2 *>(d_scoeff));
+}
+
+int s2dw_hip_plan_sync(s2dw_hip_plan_t plan) {
+  if (!plan) return kErrArgInvalid;
+  S2DW_HIP_CHECK(hipStreamSynchronize(plan->p.stream));
+  return kErrNone;
+}
+
+void *s2dw_hip_plan_stream(s2dw_hip_plan_t plan) { return plan ? (void *)plan->p.stream : nullptr; }
+
+int s2dw_hip_wigner_plane(int l, double *out, double *out2) { return wigner_plane(l, out, out2); }
+
+int s2dw_hip_plan_set_timing(s2dw_hip_plan_t plan, int enable) {
+  if (!plan) return kErrArgInvalid;
+  plan->p.timing = enable != 0;
+  return kErrNone;
+}
+
+int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count) {
+  if (!plan || !ms || !count) return kErrArgInvalid;
+  return stage_times(&plan->p, ms, count);
+}
+
+}  // extern "C"

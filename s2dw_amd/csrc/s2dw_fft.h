@@ -1,0 +1,213 @@
+// s2dw_fft.h -- LDS-resident batched FFT for gfx950.
+//
+// A workgroup transforms T sequences of length n held interleaved in LDS
+// (element k of sequence t at k*T + t, so consecutive threads touch
+// consecutive banks).  Lengths whose prime factors are in {2,3,5,7} run as
+// mixed-radix Stockham autosort stages (radix 8/4/2/3/5/7 butterflies in
+// registers, twiddles from a per-length table, ping-pong between two LDS
+// buffers with one barrier per stage).  Other lengths run through
+// Bluestein's chirp-z identity on a power-of-two work length M >= 2n-1: two
+// power-of-two transforms and a pointwise product with a precomputed filter.
+//
+// Sign convention: DIR = +1 is the unnormalised backward transform
+// sum_k x_k e^{+2 pi i jk/n}, DIR = -1 the forward transform e^{-...}.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "s2dw_internal.h"
+
+namespace s2dw {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * w (DIR>0) or a * conj(w) (DIR<0)
+template <int DIR>
+__device__ __forceinline__ double2 cmulw(double2 a, double2 w) {
+  if (DIR > 0) return make_double2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+  return make_double2(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y);
+}
+// multiply by DIR * i
+template <int DIR>
+__device__ __forceinline__ double2 cmuli(double2 a) {
+  return DIR > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+template <int R>
+struct RadixConst;
+template <>
+struct RadixConst<3> {
+  static constexpr double c[2] = {-0.5, -0.5};
+  static constexpr double s[2] = {0.86602540378443864676, -0.86602540378443864676};
+};
+template <>
+struct RadixConst<5> {
+  // cos/sin(2 pi r q / 5) for r q mod 5 = 1..4
+  static constexpr double c[4] = {0.30901699437494742410, -0.80901699437494742410, -0.80901699437494742410,
+                                  0.30901699437494742410};
+  static constexpr double s[4] = {0.95105651629515357212, 0.58778525229247312917, -0.58778525229247312917,
+                                  -0.95105651629515357212};
+};
+template <>
+struct RadixConst<7> {
+  static constexpr double c[6] = {0.62348980185873353053, -0.22252093395631440429, -0.90096886790241912624,
+                                  -0.90096886790241912624, -0.22252093395631440429, 0.62348980185873353053};
+  static constexpr double s[6] = {0.78183148246802980871, 0.97492791218182360702, 0.43388373911755812048,
+                                  -0.43388373911755812048, -0.97492791218182360702, -0.78183148246802980871};
+};
+
+// In-register DFT of odd prime length R using the symmetric pairs
+// (x_r + x_{R-r}, x_r - x_{R-r}).
+template <int R, int DIR>
+__device__ __forceinline__ void dft_odd(double2 *v) {
+  constexpr int H = (R - 1) / 2;
+  double2 ap[H], am[H];
+#pragma unroll
+  for (int r = 1; r <= H; ++r) {
+    ap[r - 1] = cadd(v[r], v[R - r]);
+    am[r - 1] = csub(v[r], v[R - r]);
+  }
+  double2 y[R];
+  double2 s0 = v[0];
+#pragma unroll
+  for (int r = 0; r < H; ++r) s0 = cadd(s0, ap[r]);
+  y[0] = s0;
+#pragma unroll
+  for (int q = 1; q <= H; ++q) {
+    double re = v[0].x, im = v[0].y, ire = 0.0, iim = 0.0;
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      const int e = (r * q) % R;  // 1..R-1
+      const double cc = RadixConst<R>::c[e - 1], ss = RadixConst<R>::s[e - 1];
+      re += ap[r - 1].x * cc;
+      im += ap[r - 1].y * cc;
+      ire += am[r - 1].x * ss;
+      iim += am[r - 1].y * ss;
+    }
+    // y_q = (re,im) + DIR*i*(ire,iim); y_{R-q} = (re,im) - DIR*i*(ire,iim)
+    const double2 base = make_double2(re, im);
+    const double2 rot = cmuli<DIR>(make_double2(ire, iim));
+    y[q] = cadd(base, rot);
+    y[R - q] = csub(base, rot);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = y[r];
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(double2 &x0, double2 &x1, double2 &x2, double2 &x3) {
+  const double2 a = cadd(x0, x2), b = csub(x0, x2), c = cadd(x1, x3), d = cmuli<DIR>(csub(x1, x3));
+  x0 = cadd(a, c);
+  x2 = csub(a, c);
+  x1 = cadd(b, d);
+  x3 = csub(b, d);
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft_small(double2 *v) {
+  if constexpr (R == 2) {
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    dft4<DIR>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 8) {
+    constexpr double h = 0.70710678118654752440;
+    double2 a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      a[r] = cadd(v[r], v[r + 4]);
+      b[r] = csub(v[r], v[r + 4]);
+    }
+    // b_r *= w^r, w = e^{DIR 2 pi i / 8}
+    b[1] = make_double2(h * (b[1].x - DIR * b[1].y), h * (b[1].y + DIR * b[1].x));
+    b[2] = cmuli<DIR>(b[2]);
+    b[3] = make_double2(h * (-b[3].x - DIR * b[3].y), h * (-b[3].y + DIR * b[3].x));
+    dft4<DIR>(a[0], a[1], a[2], a[3]);
+    dft4<DIR>(b[0], b[1], b[2], b[3]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      v[2 * q] = a[q];
+      v[2 * q + 1] = b[q];
+    }
+  } else {
+    dft_odd<R, DIR>(v);
+  }
+}
+
+// One Stockham stage of radix R over T interleaved sequences of length n.
+template <int R, int DIR>
+__device__ __forceinline__ void stockham_stage(const double2 *__restrict__ src, double2 *__restrict__ dst, int n, int logT,
+                                               int Ns, const double2 *__restrict__ tw) {
+  const int T = 1 << logT;
+  const int nb = (n / R) << logT;
+  const int twstep = n / (Ns * R);
+  for (int idx = threadIdx.x; idx < nb; idx += blockDim.x) {
+    const int t = idx & (T - 1);
+    const int j = idx >> logT;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[((j + r * (n / R)) << logT) + t];
+    const int jm = j % Ns;
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmulw<DIR>(v[r], tw[r * jm * twstep]);
+    }
+    dft_small<R, DIR>(v);
+    const int idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[((idxD + r * Ns) << logT) + t] = v[r];
+  }
+}
+
+// Runs the Stockham stages of length s.M; returns the buffer holding the result.
+template <int DIR>
+__device__ double2 *stockham(const FftSpec &s, double2 *a, double2 *b, int logT) {
+  int Ns = 1;
+  const int n = s.M;
+  for (int st = 0; st < s.nst; ++st) {
+    const int R = s.radix[st];
+    switch (R) {
+      case 8: stockham_stage<8, DIR>(a, b, n, logT, Ns, s.tw); break;
+      case 4: stockham_stage<4, DIR>(a, b, n, logT, Ns, s.tw); break;
+      case 2: stockham_stage<2, DIR>(a, b, n, logT, Ns, s.tw); break;
+      case 3: stockham_stage<3, DIR>(a, b, n, logT, Ns, s.tw); break;
+      case 5: stockham_stage<5, DIR>(a, b, n, logT, Ns, s.tw); break;
+      default: stockham_stage<7, DIR>(a, b, n, logT, Ns, s.tw); break;
+    }
+    __syncthreads();
+    double2 *t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+// Full length-n transform of the T sequences in A (k < n).  B is the second
+// LDS buffer (both hold M*T elements).  Returns the buffer with the result.
+// Caller must __syncthreads() after filling A.
+template <int DIR>
+__device__ double2 *fft_exec(const FftSpec &s, double2 *A, double2 *B, int logT) {
+  if (!s.bluestein) return stockham<DIR>(s, A, B, logT);
+  const int n = s.n, M = s.M, T = 1 << logT;
+  for (int idx = threadIdx.x; idx < (M << logT); idx += blockDim.x) {
+    const int k = idx >> logT;
+    A[idx] = k < n ? cmulw<DIR>(A[idx], s.chirp[k]) : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double2 *X = stockham<-1>(s, A, B, logT);
+  const double2 *F = DIR > 0 ? s.filt_b : s.filt_f;
+  for (int idx = threadIdx.x; idx < (M << logT); idx += blockDim.x) X[idx] = cmul(X[idx], F[idx >> logT]);
+  __syncthreads();
+  double2 *Y = stockham<+1>(s, X, X == A ? B : A, logT);
+  for (int idx = threadIdx.x; idx < (n << logT); idx += blockDim.x) Y[idx] = cmulw<DIR>(Y[idx], s.chirp[idx >> logT]);
+  __syncthreads();
+  (void)T;
+  return Y;
+}
+
+}  // namespace s2dw

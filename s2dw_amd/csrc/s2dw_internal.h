@@ -1,0 +1,146 @@
+// s2dw_internal.h -- declarations shared by the host and device sources of
+// libs2dw_hip.so.  Not part of the public C ABI (include/s2dw_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace s2dw {
+
+// Error codes, numbering of s2dw_error_mod.f90:37-52.
+enum {
+  kErrNone = 0,
+  kErrInit = 1,
+  kErrNotInit = 2,
+  kErrInitFail = 3,
+  kErrMemAlloc = 4,
+  kErrArth = 5,
+  kErrSizeInvalid = 8,
+  kErrArgInvalid = 10,
+  kErrQuadStepExceed = 12,
+  kErrAdmissFail = 13,
+};
+
+// Host setup (s2dw_setup.cpp).
+double powi(double a, int b);
+bool assimilate_int(double x, int *y);
+int comp_Jmax(int B, double alpha);
+bool params_valid(int J, int B, double alpha, int N);
+void band_limits(int jj, int B, double alpha, int *hi, int *lo);
+int init_kernels(double *K, double *Phi2, int *bl_scoeff, int J, int B, double alpha);
+int init_directionality(double *S, int B, int N);
+bool admiss(double *a, const double *K, const double *Phi2, int B, int J);
+double quad_weight(double theta, int B);
+
+// Error text for the calling thread (s2dw_capi.cpp).
+void set_error(const char *fmt, ...);
+int hip_error(hipError_t e, const char *what);
+
+#define S2DW_HIP_CHECK(call)                                  \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return ::s2dw::hip_error(e_, #call); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// FFT description passed by value to kernels.  A length-n transform is run
+// either directly as mixed-radix Stockham stages (n = product of radices in
+// {2,3,4,5,7,8}) or, when n has a larger prime factor, by Bluestein's chirp-z
+// convolution on a power-of-two length M >= 2n-1.
+// ---------------------------------------------------------------------------
+constexpr int kMaxStages = 16;
+struct FftSpec {
+  int n;          // logical length
+  int M;          // work length (n, or the Bluestein power of two)
+  int bluestein;  // 0/1
+  int nst;        // number of Stockham stages for M
+  int radix[kMaxStages];
+  const double2 *tw;     // M entries, e^{+2 pi i k / M}
+  const double2 *chirp;  // n entries, e^{+i pi (k^2 mod 2n) / n}       (Bluestein)
+  const double2 *filt_b; // M entries, FFT_M(conj chirp) / M  (backward transforms)
+  const double2 *filt_f; // M entries, FFT_M(chirp) / M       (forward transforms)
+};
+
+// Per-scale geometry.
+struct Scale {
+  int hi, lo;            // bl_hi, bl_lo
+  long long wav_off;     // offset (doubles) of this scale in the packed wav
+  long long tg_off;      // offset (double2) of Tg_j / Vt_j in the contraction buffer
+  long long tb_off;      // offset (double2) of Tb_j / V_j in the transform buffer
+  int fa, fb;            // FFT slots: alpha (2hi-1) and beta (2hi)
+};
+
+// Columns of the real B matrices of the l-contraction: one per (n, part of
+// S(l,n)) that is not identically zero.
+struct Column {
+  int n;     // azimuthal index (analysis: |n| <= N-1 with N+n odd; synthesis n >= 0)
+  int part;  // 0: real part of S, 1: imaginary part
+};
+
+// Optional per-stage HIP-event timing (stage ids in include/s2dw_hip.h).
+constexpr int kNumStages = 9;
+struct StageEvents {
+  int stage;
+  hipEvent_t a, b;
+};
+
+struct Plan {
+  int B = 0, N = 0, J = 0, bl_scoeff = 0, precision = 0, device = 0;
+  double alpha = 0;
+  std::vector<Scale> sc;
+  std::vector<Column> cols_a, cols_s;
+  int nk = 0;  // number of thin Delta^l_{m'k} chains per m' (k = 0..nk-1)
+  long long wav_doubles = 0;
+  std::vector<double> K, Phi2, S;  // host copies (reference layouts)
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // device tables
+  double *d_S0 = nullptr;     // chain start values, triangle [l0][k]
+  double *d_thin = nullptr;   // Delta^l_{m'k}, [tri(m',l)][k]
+  double *d_Ba = nullptr;     // analysis rows [tri(m',l)][2 + 2*NCa]
+  double *d_Bs = nullptr;     // synthesis rows [tri(l,m')][2 + 2*NCs]
+  int *d_lmeta = nullptr;     // per l: flush info
+  int4 *d_sblk = nullptr;     // synthesis l-blocks (l_first, count, slot-0 scale, slot-1 scale)
+  int n_sblk = 0;
+  double2 *d_G = nullptr;     // analysis gamma matrix [g][c]
+  double *d_wq = nullptr;     // Driscoll-Healy weights, per scale 2hi (offset = tb-like)
+  long long *d_wq_off = nullptr;
+  std::vector<long long> wq_off;
+  std::vector<FftSpec> ffts;  // distinct lengths
+  std::vector<void *> allocs; // every device allocation, freed on destroy
+  // scratch
+  double2 *d_fT = nullptr;    // [l][m]
+  double2 *d_tg = nullptr;    // contraction output / synthesis Vt
+  double2 *d_tb = nullptr;    // beta/alpha intermediate
+  double2 *d_fo = nullptr;    // synthesis output [l][m]
+  double2 *d_tw8 = nullptr;   // per scale e^{2 pi i k/(8hi)} tables
+  std::vector<long long> tw8_off;
+  double2 *eg = nullptr;      // synthesis gamma table [n-index][g]
+  std::vector<int> nlist;     // synthesis n values (n >= 0, N+n odd)
+  unsigned char *d_cov = nullptr;  // per l: covered by some non-empty scale
+  double *d_Phi2 = nullptr;
+  int *d_his = nullptr;       // per scale bl_hi
+  long long *d_woffs = nullptr;  // per scale packed wav offset
+  long long tg_doubles = 0;    // size of d_tg in doubles
+  long long device_bytes = 0;
+  bool timing = false;
+  std::vector<StageEvents> ev_live;  // recorded, not yet read
+  std::vector<hipEvent_t> ev_pool;   // free events
+  int syn_q = 4;              // chains per lane in the synthesis contraction
+  int ana_p = 2;              // chains per lane in the analysis contraction
+};
+
+int plan_build(Plan *p, int B, int N, int J, double alpha, const double *K, const double *Phi2, const double *S,
+               int precision, hipStream_t stream, bool own_stream);
+void plan_free(Plan *p);
+int run_analysis(Plan *p, double *d_wav, double2 *d_scoeff, const double2 *d_flm);
+int run_synthesis(Plan *p, double2 *d_flm, const double *d_wav, const double2 *d_scoeff);
+int stage_times(Plan *p, double *ms, int *count);
+int pack_static(Plan *p, const double *d_packed, double *d_static, bool to_static);
+int wigner_plane(int l, double *out, double *out2);
+
+}  // namespace s2dw

@@ -1,0 +1,285 @@
+"""Pins of the CPU oracle (oracle/s2dw_oracle.c) -- CPU only.
+
+Each oracle routine is checked against something that does not share its
+code: golden vectors written by the reference's own Fortran (compiled from its
+sources into oracle/_ref, fixtures in tests/golden), textbook closed forms and
+identities, the independent definition of the transform (a brute-force sum of
+Wigner D-functions with d^l(beta) = exp(-i beta J_y)), scipy's spherical
+harmonics (N = 1), numpy's FFT, and the exact round trip.
+"""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+import oracle as O
+from conftest import seeded_flm
+
+
+# ------------------------------------------------------------- setup routines
+def test_kernels_match_reference(golden):
+    g = golden("ref_kernels.npz")
+    keys = sorted({k.rsplit("_", 1)[0] for k in g.files if k.endswith("_meta")})
+    assert len(keys) >= 10
+    for key in keys:
+        B, J, jmax, bl, ok = (int(x) for x in g[key + "_meta"])
+        alpha = float(g[key + "_alpha"][0])
+        K, Phi2, bl2 = O.init_kernels(B, alpha, J)
+        assert bl2 == bl, key
+        np.testing.assert_array_max_ulp(K, g[key + "_K"], maxulp=2)
+        np.testing.assert_array_max_ulp(Phi2, g[key + "_Phi2"], maxulp=2)
+        a, ok2 = O.admiss(K, Phi2, B, J)
+        assert ok2 == bool(ok), key
+        np.testing.assert_allclose(a, g[key + "_admiss"], rtol=0, atol=1e-15)
+        assert O.comp_Jmax(B, alpha) == jmax
+        hi, lo = O.band_limits(J, B, alpha)
+        np.testing.assert_array_equal(np.stack([hi, lo], 1), g[key + "_bands"])
+
+
+def test_jmax_sweep_matches_reference(golden):
+    for B, alpha, jm in golden("ref_jmax.npy"):
+        assert O.comp_Jmax(int(B), float(alpha)) == int(jm), (B, alpha)
+
+
+def test_band_limits_table_c3():
+    hi, lo = O.band_limits(7, 512, 2.0)
+    assert hi == [512, 512, 256, 128, 64, 32, 16, 8]
+    assert lo == [256, 128, 64, 32, 16, 8, 4, 2]
+
+
+def test_admissibility_identity():
+    # Phi^2 + sum_j K_j^2 = 1 (resolution of identity, s2dw_core_mod.f90:285-304)
+    for B, alpha in ((64, 2.0), (81, 3.0), (100, 2.0)):
+        J = O.comp_Jmax(B, alpha)
+        K, Phi2, _ = O.init_kernels(B, alpha, J)
+        a, ok = O.admiss(K, Phi2, B, J)
+        assert ok and np.abs(a - 1).max() < 1e-12
+
+
+def test_directionality_matches_reference(golden):
+    g = golden("ref_direc.npz")
+    for key in g.files:
+        B, N = (int(x[1:]) for x in key.split("_"))
+        np.testing.assert_array_equal(O.init_directionality(B, N), g[key])
+
+
+def test_directionality_closed_form():
+    # S(l,n) = sqrt(C(g,(g-n)/2)/2^g) eta for N+n odd, g = min(N-1, l-(1+(-1)^(N+l))/2);
+    # for l >= N-1 the weights satisfy sum_{n=-g..g} |S(l,|n|)|^2 = 1.
+    from math import comb
+    for N in (1, 2, 3, 4, 5, 8):
+        S = O.init_directionality(40, N)
+        for l in range(1, 40):
+            g = min(N - 1, l - (1 + (-1) ** (N + l)) // 2)
+            for n in range(N):
+                if (N + n) % 2 and n <= min(l, N - 1):
+                    want = np.sqrt(comb(g, (g - n) // 2) / 2.0 ** g)
+                    assert abs(abs(S[l, n]) - want) < 1e-15
+                else:
+                    assert S[l, n] == 0
+            if l >= N - 1:
+                tot = sum(abs(S[l, abs(n)]) ** 2 for n in range(-(N - 1), N) if (N + n) % 2)
+                assert abs(tot - 1) < 1e-14
+
+
+# ------------------------------------------------------------- Wigner planes
+def wigner_explicit(l, mp, m, beta):
+    """Wigner's explicit sum (e.g. Sakurai eq. 3.8.33) in 50-digit arithmetic."""
+    from mpmath import mp as M, mpf, factorial, sqrt, cos, sin
+    M.dps = 50
+    b = mpf(beta)
+    tot = mpf(0)
+    for s in range(max(0, m - mp), min(l + m, l - mp) + 1):
+        tot += ((-1) ** (mp - m + s) * cos(b / 2) ** (2 * l + m - mp - 2 * s) * sin(b / 2) ** (mp - m + 2 * s)
+                / (factorial(l + m - s) * factorial(s) * factorial(mp - m + s) * factorial(l - mp - s)))
+    return float(sqrt(factorial(l + mp) * factorial(l - mp) * factorial(l + m) * factorial(l - m)) * tot)
+
+
+def test_dl_matches_reference_full_planes(golden):
+    g = golden("ref_dl.npz")
+    for key in [k for k in g.files if not k.startswith("big")]:
+        l = int(key[1:])
+        np.testing.assert_allclose(O.dl_beta_operator(l), g[key], rtol=0, atol=1e-15)
+
+
+def test_dl_matches_reference_large_l(golden):
+    g = golden("ref_dl.npz")
+    for l in (100, 127, 255, 256, 511, 1023):
+        P = O.dl_beta_operator(l)
+        Q = P[l:, l:]
+        np.testing.assert_allclose(Q[:, :9], g[f"big{l}_cols"], rtol=0, atol=1e-15)
+        np.testing.assert_allclose(Q[g[f"big{l}_rowidx"], :], g[f"big{l}_rows"], rtol=0, atol=1e-15)
+        idx = g[f"big{l}_sidx"]
+        np.testing.assert_allclose(Q[idx[:, 0], idx[:, 1]], g[f"big{l}_svals"], rtol=0, atol=1e-15)
+
+
+def test_dl_generic_beta_matches_reference(golden):
+    g = golden("ref_dl_beta.npz")
+    for key in g.files:
+        l = int(key.split("_")[0][1:])
+        beta = float(key.split("_b")[1])
+        np.testing.assert_allclose(O.dl_beta_operator(l, beta), g[key], rtol=0, atol=1e-14)
+
+
+def test_dl_explicit_formula():
+    for l in (1, 2, 3, 5, 8):
+        P = O.dl_beta_operator(l)
+        for mp in range(-l, l + 1):
+            for m in range(-l, l + 1):
+                assert abs(P[mp + l, m + l] - wigner_explicit(l, mp, m, np.pi / 2)) < 1e-14
+    # generic beta
+    P = O.dl_beta_operator(6, 0.7)
+    for mp in (-6, -2, 0, 3, 6):
+        for m in (-5, 0, 1, 6):
+            assert abs(P[mp + 6, m + 6] - wigner_explicit(6, mp, m, 0.7)) < 1e-14
+
+
+def test_dl_identities():
+    from scipy.special import eval_legendre
+    for l in (7, 40, 129):
+        P = O.dl_beta_operator(l)
+        # orthogonality: sum_m' D_{m'm} D_{m'n} = delta_mn
+        np.testing.assert_allclose(P.T @ P, np.eye(2 * l + 1), atol=1e-12)
+        # d_{m'm} = (-1)^{m-m'} d_{mm'} = d_{-m,-m'}
+        sgn = (-1.0) ** (np.arange(-l, l + 1)[None, :] - np.arange(-l, l + 1)[:, None])
+        np.testing.assert_allclose(P, sgn * P.T, atol=1e-13)
+        np.testing.assert_allclose(P, P[::-1, ::-1].T, atol=1e-13)
+        # d^l_00(pi/2) = P_l(0)
+        assert abs(P[l, l] - eval_legendre(l, 0.0)) < 1e-13
+
+
+# ------------------------------------------------------------- transforms
+def wigner_d_expm(l, beta):
+    n = 2 * l + 1
+    ms = np.arange(-l, l + 1)
+    Jp = np.zeros((n, n))
+    for i, m in enumerate(ms[:-1]):
+        Jp[i + 1, i] = np.sqrt((l - m) * (l + m + 1))
+    return expm(-1j * beta * (Jp - Jp.T) / 2j).real
+
+
+def test_expm_wigner_convention():
+    for l in (1, 3, 6):
+        d = wigner_d_expm(l, 1.1)
+        for mp in range(-l, l + 1, 2):
+            for m in range(-l, l + 1):
+                assert abs(d[mp + l, m + l] - wigner_explicit(l, mp, m, 1.1)) < 1e-13
+
+
+def brute_analysis(st, flm, j):
+    """W_j(alpha_a, beta_b, gamma_g) = sum_{l,m,n} f_lm psi*_{j,ln} e^{i m alpha} d^l_mn(beta) e^{i n gamma}:
+    the directional wavelet coefficient <f, R_(alpha,beta,gamma) Psi_j> written as a sum of
+    Wigner D-functions, with f_{l,-m} = (-1)^m conj f_lm and psi_{j,l,-n} = (-1)^n conj psi_{j,ln}."""
+    N = st.N
+    hi, lo = st.hi[j], st.lo[j]
+    na = 2 * hi - 1
+    al = 2 * np.pi * np.arange(na) / na
+    be = np.pi * (2 * np.arange(2 * hi) + 1) / (4 * hi)
+    ga = np.pi * np.arange(N) / N
+    W = np.zeros((na, 2 * hi, N), complex)
+    ns = [n for n in range(-(N - 1), N) if (N + n) % 2 == 1]
+    for l in range(lo + 1, hi):
+        ds = [wigner_d_expm(l, b) for b in be]
+        for m in range(-l, l + 1):
+            f = flm[l, m] if m >= 0 else (-1) ** m * np.conj(flm[l, -m])
+            for n in ns:
+                if abs(n) > l:
+                    continue
+                psi = st.K[j, l] * np.conj(st.S[l, n]) if n >= 0 else (-1) ** n * st.K[j, l] * st.S[l, -n]
+                ea = np.exp(1j * m * al)[:, None]
+                eg = np.exp(1j * n * ga)[None, :]
+                for bi, d in enumerate(ds):
+                    W[:, bi, :] += f * psi * ea * d[m + l, n + l] * eg
+    return W
+
+
+@pytest.mark.parametrize("B,N,alpha", [(8, 3, 2.0), (8, 2, 2.0), (9, 1, 3.0), (8, 4, 2.0), (10, 5, 2.0)])
+def test_analysis_brute_force(B, N, alpha):
+    st = O.Setup(B, N, alpha)
+    f = seeded_flm(B, 11)
+    w, sc = st.analysis(f)
+    for j in range(st.J + 1):
+        Wb = brute_analysis(st, f, j)
+        assert np.abs(Wb.imag).max() < 1e-13
+        np.testing.assert_allclose(w[j], Wb.real, rtol=0, atol=1e-13)
+    np.testing.assert_array_equal(sc, f[:st.bl_scoeff, :st.bl_scoeff])
+
+
+def test_analysis_n1_is_spherical_harmonic_synthesis():
+    # N = 1: S(l,0) = 1 (l >= 1), so W_j(alpha,beta) = sum_lm K_j(l) f_lm sqrt(4pi/(2l+1)) Y_lm(beta, alpha)
+    from scipy.special import sph_harm_y
+    B = 16
+    st = O.Setup(B, 1, 2.0)
+    f = seeded_flm(B, 5)
+    w, _ = st.analysis(f)
+    for j in range(st.J + 1):
+        hi, lo = st.hi[j], st.lo[j]
+        na = 2 * hi - 1
+        a = 2 * np.pi * np.arange(na) / na
+        b = np.pi * (2 * np.arange(2 * hi) + 1) / (4 * hi)
+        A, Bt = np.meshgrid(a, b, indexing="ij")
+        W = np.zeros_like(A, dtype=complex)
+        for l in range(max(lo + 1, 1), hi):
+            c = st.K[j, l] * np.sqrt(4 * np.pi / (2 * l + 1))
+            for m in range(-l, l + 1):
+                flm = f[l, m] if m >= 0 else (-1) ** m * np.conj(f[l, -m])
+                W += c * flm * sph_harm_y(l, m, Bt, A)
+        np.testing.assert_allclose(w[j][:, :, 0], W.real, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("B,N,alpha,J", [(32, 3, 2.0, None), (32, 8, 2.0, None), (27, 4, 3.0, None),
+                                         (64, 3, 2.0, None), (40, 5, 2.0, 3), (33, 2, 2.0, None)])
+def test_round_trip(B, N, alpha, J):
+    st = O.Setup(B, N, alpha, J=J)
+    f = seeded_flm(B, B + N)
+    w, sc = st.analysis(f)
+    g = st.synthesis(w, sc)
+    assert np.abs(g - f).max() < 1e-12
+
+
+def test_static_and_dynamic_layouts_agree():
+    st = O.Setup(24, 3, 2.0)
+    f = seeded_flm(24, 3)
+    w, sc = st.analysis(f)
+    ws, sc2 = O.analysis_static(f, st.K, st.S, st.J, st.B, st.N, st.bl_scoeff, st.alpha)
+    for j, wj in enumerate(w):
+        h = st.hi[j]
+        np.testing.assert_array_equal(ws[j, :2 * h - 1, :2 * h, :], wj)
+    g1 = st.synthesis(w, sc)
+    g2 = O.synthesis_static(ws, sc2, st.K, st.Phi2, st.S, st.J, st.B, st.N, st.bl_scoeff, st.alpha)
+    np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-14)
+
+
+def test_linearity():
+    st = O.Setup(24, 5, 2.0)
+    f1, f2 = seeded_flm(24, 1), seeded_flm(24, 2)
+    w1, _ = st.analysis(f1)
+    w2, _ = st.analysis(f2)
+    w3, _ = st.analysis(2.5 * f1 - f2)
+    for a, b, c in zip(w1, w2, w3):
+        np.testing.assert_allclose(c, 2.5 * a - b, rtol=0, atol=1e-12)
+
+
+def test_fft_matches_numpy():
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 7, 15, 31, 63, 127, 255, 511, 1023, 1024):
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        np.testing.assert_allclose(O.fft(x, -1), np.fft.fft(x), rtol=0, atol=1e-11 * np.sqrt(n))
+        np.testing.assert_allclose(O.fft(x, +1), np.fft.ifft(x) * n, rtol=0, atol=1e-11 * np.sqrt(n))
+
+
+def test_quad_weights_integrate_exactly():
+    # Driscoll-Healy weights integrate band-limited polynomials in cos(theta) exactly:
+    # sum_b w_b P_k(cos beta_b) = 2 delta_k0 for k < 2B.
+    from scipy.special import eval_legendre
+    for B in (4, 9, 16):
+        be = np.pi * (2 * np.arange(2 * B) + 1) / (4 * B)
+        w = np.array([O.quad_weights(b, B) for b in be])
+        for k in range(2 * B):
+            assert abs((w * eval_legendre(k, np.cos(be))).sum() - (2.0 if k == 0 else 0.0)) < 1e-13
+
+
+def test_ran2_test_signal_is_deterministic():
+    f1, f2 = O.test_gen_flm(16, 1), O.test_gen_flm(16, 1)
+    np.testing.assert_array_equal(f1, f2)
+    assert np.all(np.abs(f1.real) <= 1) and np.all(np.triu(f1, 1) == 0)
+    assert np.all(f1.imag[:, 0] == 0)
