@@ -75,6 +75,14 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m s2dw_amd.build`")
+        # torch's wheel carries its own HIP runtime (soname libamdhip64.so.7).
+        # Loading torch first makes this library bind to that same runtime by
+        # soname; loaded the other way round, the process would hold two HIP
+        # runtimes and torch would find no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.s2dw_hip_last_error.restype = C.c_char_p
         L.s2dw_hip_version.restype = C.c_char_p

@@ -210,4 +210,138 @@ __device__ double2 *fft_exec(const FftSpec &s, double2 *A, double2 *B, int logT)
   return Y;
 }
 
+// ---------------------------------------------------------------------------
+// Wave-level power-of-two FFT.  A wave owns Tw = 64*E/n interleaved sequences
+// of length n (element (k,t) at k*Tw + t of the wave's LDS region of 64*E
+// elements).  Each Stockham stage: every lane loads its E/R radix-R
+// butterflies into registers, twiddles, transforms, and writes them back in
+// place.  A wave's LDS operations complete in program order, so no barrier is
+// needed between stages; only the wave's own region is touched.
+// ---------------------------------------------------------------------------
+template <int DIR>
+__device__ __forceinline__ void dft16(double2 *v) {
+  // 16 = 4 x 4: radix-4 over stride-4 groups, twiddles w16^{q r}, radix-4 again
+  double2 a[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[r][q] = v[r + 4 * q];
+    dft4<DIR>(a[r][0], a[r][1], a[r][2], a[r][3]);
+  }
+  constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+  // a[r][q] *= w16^{r q}, w16 = e^{DIR 2 pi i/16}
+#pragma unroll
+  for (int r = 1; r < 4; ++r)
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+      const int e = r * q;  // 1..9
+      double cr, ci;
+      switch (e) {
+        case 1: cr = c1; ci = s1; break;
+        case 2: cr = h; ci = h; break;
+        case 3: cr = s1; ci = c1; break;
+        case 4: cr = 0.0; ci = 1.0; break;
+        case 6: cr = -h; ci = h; break;
+        default: cr = -c1; ci = -s1; break;  // e = 9: w^9 = (-c1, -s1)
+      }
+      const double2 x = a[r][q];
+      a[r][q] = make_double2(x.x * cr - DIR * x.y * ci, x.y * cr + DIR * x.x * ci);
+    }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    dft4<DIR>(a[0][q], a[1][q], a[2][q], a[3][q]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[q + 4 * r] = a[r][q];
+  }
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft_pow2(double2 *v) {
+  if constexpr (R == 16) dft16<DIR>(v);
+  else dft_small<R, DIR>(v);
+}
+
+template <int R, int E, int DIR>
+__device__ __forceinline__ void wave_stage(double2 *__restrict__ buf, int n, int logTw, int Ns,
+                                           const double2 *__restrict__ tw, int lane) {
+  constexpr int NB = E / R;
+  const int Tw = 1 << logTw;
+  const int nR = n / R;
+  const int twstep = n / (Ns * R);
+  double2 v[NB][R];
+  int jj[NB], tt[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int b = lane + 64 * q;
+    tt[q] = b & (Tw - 1);
+    jj[q] = b >> logTw;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q][r] = buf[((jj[q] + r * nR) << logTw) + tt[q]];
+  }
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int jm = jj[q] & (Ns - 1);
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[q][r] = cmulw<DIR>(v[q][r], tw[r * jm * twstep]);
+    }
+    dft_pow2<R, DIR>(v[q]);
+    const int idxD = (jj[q] - jm) * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf[((idxD + r * Ns) << logTw) + tt[q]] = v[q][r];
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Radix plan for a power of two n <= 64*E: as many 16s as possible, then 8/4/2.
+template <int E, int DIR>
+__device__ __forceinline__ void wave_fft(double2 *buf, int n, int logTw, const double2 *__restrict__ tw) {
+  const int lane = threadIdx.x & 63;
+  int Ns = 1, rem = n;
+  while (rem > 1) {
+    if (rem % 16 == 0 && E % 16 == 0) {
+      wave_stage<16, E, DIR>(buf, n, logTw, Ns, tw, lane);
+      Ns *= 16;
+      rem /= 16;
+    } else if (rem % 8 == 0 && E % 8 == 0) {
+      wave_stage<8, E, DIR>(buf, n, logTw, Ns, tw, lane);
+      Ns *= 8;
+      rem /= 8;
+    } else if (rem % 4 == 0) {
+      wave_stage<4, E, DIR>(buf, n, logTw, Ns, tw, lane);
+      Ns *= 4;
+      rem /= 4;
+    } else {
+      wave_stage<2, E, DIR>(buf, n, logTw, Ns, tw, lane);
+      Ns *= 2;
+      rem /= 2;
+    }
+  }
+}
+
+// Length-n transform (direct power of two, or Bluestein on a power-of-two M)
+// of the Tw sequences in the wave's region (k < n filled).  Result in place.
+template <int E, int DIR>
+__device__ __forceinline__ void wave_fft_exec(const FftSpec &s, double2 *buf, int logTw) {
+  const int lane = threadIdx.x & 63;
+  if (!s.bluestein) {
+    wave_fft<E, DIR>(buf, s.M, logTw, s.tw);
+    return;
+  }
+  const int n = s.n, M = s.M;
+  for (int idx = lane; idx < (M << logTw); idx += 64) {
+    const int k = idx >> logTw;
+    buf[idx] = k < n ? cmulw<DIR>(buf[idx], s.chirp[k]) : make_double2(0.0, 0.0);
+  }
+  __builtin_amdgcn_wave_barrier();
+  wave_fft<E, -1>(buf, M, logTw, s.tw);
+  const double2 *F = DIR > 0 ? s.filt_b : s.filt_f;
+  for (int idx = lane; idx < (M << logTw); idx += 64) buf[idx] = cmul(buf[idx], F[idx >> logTw]);
+  __builtin_amdgcn_wave_barrier();
+  wave_fft<E, +1>(buf, M, logTw, s.tw);
+  for (int idx = lane; idx < (n << logTw); idx += 64) buf[idx] = cmulw<DIR>(buf[idx], s.chirp[idx >> logTw]);
+  __builtin_amdgcn_wave_barrier();
+}
+
 }  // namespace s2dw

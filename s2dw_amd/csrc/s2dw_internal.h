@@ -103,7 +103,8 @@ struct Plan {
   double *d_thin = nullptr;   // Delta^l_{m'k}, [tri(m',l)][k]
   double *d_Ba = nullptr;     // analysis rows [tri(m',l)][2 + 2*NCa]
   double *d_Bs = nullptr;     // synthesis rows [tri(l,m')][2 + 2*NCs]
-  int *d_lmeta = nullptr;     // per l: flush info
+  int *d_lmeta = nullptr;     // per l: flush info (int2), then next flush l (int)
+  int *d_nextev = nullptr;    // inside d_lmeta
   int4 *d_sblk = nullptr;     // synthesis l-blocks (l_first, count, slot-0 scale, slot-1 scale)
   int n_sblk = 0;
   double2 *d_G = nullptr;     // analysis gamma matrix [g][c]
@@ -125,13 +126,15 @@ struct Plan {
   double *d_Phi2 = nullptr;
   int *d_his = nullptr;       // per scale bl_hi
   long long *d_woffs = nullptr;  // per scale packed wav offset
+  int max_lds = 65536;         // LDS bytes a workgroup may use on this device
   long long tg_doubles = 0;    // size of d_tg in doubles
   long long device_bytes = 0;
   bool timing = false;
   std::vector<StageEvents> ev_live;  // recorded, not yet read
   std::vector<hipEvent_t> ev_pool;   // free events
   int syn_q = 4;              // chains per lane in the synthesis contraction
-  int ana_p = 2;              // chains per lane in the analysis contraction
+  int ana_p = 2;              // chains per lane in the VALU analysis contraction
+  bool ana_mfma = true;       // analysis contraction on FP64 MFMA
 };
 
 int plan_build(Plan *p, int B, int N, int J, double alpha, const double *K, const double *Phi2, const double *S,
