@@ -344,4 +344,220 @@ __device__ __forceinline__ void wave_fft_exec(const FftSpec &s, double2 *buf, in
   __builtin_amdgcn_wave_barrier();
 }
 
+// ---------------------------------------------------------------------------
+// Workgroup FFT engine.  A workgroup of NT threads owns S = NT*E complex LDS
+// slots holding Tw = S/M interleaved sequences of power-of-two length M
+// (logical slot k*Tw + t holds element k of sequence t).  The physical LDS
+// index of logical slot i is i + i/16: one pad slot per 16 spreads the
+// stride-R writes of the early Stockham stages over the LDS banks.
+//
+// Stage (radix R, Ns = product of the radices before it): the q-th
+// butterfly b = threadIdx.x + NT*q, (jj, tt) = (b / Tw, b % Tw), reads
+// elements jj + r*M/R of sequence tt, twiddles them by w^{r*(jj mod Ns)},
+// w = e^{DIR 2 pi i/(Ns R)}, transforms, and writes them to
+// (jj - jj mod Ns)*R + jj mod Ns + r*Ns.  Every LDS read and every global
+// load (twiddles, optional post-multiplier) of a stage is issued before the
+// barrier that separates the reads from the in-place writes.  Twiddles come
+// from a table of e^{+2 pi i k/M} (k < M) as the powers w, w^2, w^4, w^8;
+// the others are products of those (at most three roundings).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lpad_size(int s) { return s + (s >> 4); }
+
+template <int R, int DIR>
+__device__ __forceinline__ void twiddle_powers(double2 *v, double2 w1, double2 w2, double2 w4, double2 w8) {
+  // v[r] *= w^r for r = 1..R-1 (w^r with the sign of DIR)
+  double2 p[R];
+  p[1] = w1;
+  if constexpr (R > 2) {
+    p[2] = w2;
+    p[3] = cmul(w1, w2);
+  }
+  if constexpr (R > 4) {
+    p[4] = w4;
+    p[5] = cmul(w1, w4);
+    p[6] = cmul(w2, w4);
+    p[7] = cmul(p[3], w4);
+  }
+  if constexpr (R > 8) {
+    p[8] = w8;
+#pragma unroll
+    for (int r = 9; r < 16; ++r) p[r] = cmul(p[r - 8], w8);
+  }
+#pragma unroll
+  for (int r = 1; r < R; ++r) v[r] = cmulw<DIR>(v[r], p[r]);
+}
+
+template <int R, int E, int NT, int DIR>
+__device__ __forceinline__ void team_stage(double2 *buf, int M, int logTw, int Ns, const double2 *__restrict__ tw,
+                                           const double2 *__restrict__ post) {
+  constexpr int NB = E / R;
+  const int Tw = 1 << logTw;
+  const int nR = M / R;
+  const int twstep = M / (Ns * R);
+  double2 v[NB][R];
+  double2 w1[NB], w2[NB], w4[NB], w8[NB];
+  double2 pm[NB][R];
+  int tt[NB], idxD[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int b = threadIdx.x + NT * q;
+    tt[q] = b & (Tw - 1);
+    const int jj = b >> logTw;
+    const int jm = jj & (Ns - 1);
+    idxD[q] = (jj - jm) * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q][r] = buf[lpad(((jj + r * nR) << logTw) + tt[q])];
+    if (Ns > 1) {
+      const int e = jm * twstep;
+      w1[q] = tw[e];
+      if (R > 2) w2[q] = tw[2 * e];
+      if (R > 4) w4[q] = tw[4 * e];
+      if (R > 8) w8[q] = tw[8 * e];
+    }
+    if (post) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) pm[q][r] = post[idxD[q] + r * Ns];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    if (Ns > 1) twiddle_powers<R, DIR>(v[q], w1[q], w2[q], w4[q], w8[q]);
+    dft_pow2<R, DIR>(v[q]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double2 y = v[q][r];
+      if (post) y = cmul(y, pm[q][r]);
+      buf[lpad(((idxD[q] + r * Ns) << logTw) + tt[q])] = y;
+    }
+  }
+  __syncthreads();
+}
+
+// Radix-8 stage with its twiddles (w, w^2, w^4 per butterfly) and, for the
+// last stage of a Bluestein forward transform, the filter values already in
+// registers.
+template <int E, int NT, int DIR>
+__device__ __forceinline__ void team_stage8(double2 *buf, int M, int logTw, int Ns, const double2 (&w)[E / 8][3],
+                                            const double2 (&pm)[E / 8][8], bool use_pm) {
+  constexpr int R = 8, NB = E / R;
+  const int Tw = 1 << logTw;
+  const int nR = M / R;
+  double2 v[NB][R];
+  int tt[NB], idxD[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int b = threadIdx.x + NT * q;
+    tt[q] = b & (Tw - 1);
+    const int jj = b >> logTw;
+    const int jm = jj & (Ns - 1);
+    idxD[q] = (jj - jm) * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q][r] = buf[lpad(((jj + r * nR) << logTw) + tt[q])];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    if (Ns > 1) twiddle_powers<R, DIR>(v[q], w[q][0], w[q][1], w[q][2], w[q][2]);
+    dft_pow2<R, DIR>(v[q]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double2 y = v[q][r];
+      if (use_pm) y = cmul(y, pm[q][r]);
+      buf[lpad(((idxD[q] + r * Ns) << logTw) + tt[q])] = y;
+    }
+  }
+  __syncthreads();
+}
+
+// Power-of-two transform of all Tw sequences in the workgroup's slots, in
+// place: a first stage of radix R0 = 2^(log2 M mod 3) (no twiddles), then
+// radix-8 stages whose twiddles (at most 3 stages for M <= 2048) are loaded
+// once, up front, by team_twiddles.
+template <int E>
+struct TeamTw {
+  double2 w[3][E / 8][3];
+};
+
+__device__ __forceinline__ int ilog2_dev(int x) { return 31 - __clz(x); }
+
+template <int E, int NT>
+__device__ __forceinline__ void team_twiddles(TeamTw<E> &T, int M, int logTw, const double2 *__restrict__ tw) {
+  const int logM = ilog2_dev(M), nst = logM / 3;
+  int Ns = 1 << (logM % 3);
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    if (s < nst) {
+      const int step = M / (Ns * 8);
+#pragma unroll
+      for (int q = 0; q < E / 8; ++q) {
+        const int jj = (threadIdx.x + NT * q) >> logTw;
+        const int e = (jj & (Ns - 1)) * step;
+        T.w[s][q][0] = tw[e];
+        T.w[s][q][1] = tw[2 * e];
+        T.w[s][q][2] = tw[4 * e];
+      }
+    }
+    Ns *= 8;
+  }
+}
+
+template <int E, int NT, int DIR>
+__device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const TeamTw<E> &T,
+                                         const double2 *__restrict__ tw, const double2 *__restrict__ post,
+                                         const double2 (&pm)[E / 8][8], bool use_pm) {
+  static_assert(E % 8 == 0, "radix-8 engine");
+  const int logM = ilog2_dev(M), nst = logM / 3, r0 = logM % 3;
+  if (r0 == 1) team_stage<2, E, NT, DIR>(buf, M, logTw, 1, tw, nst == 0 ? post : nullptr);
+  else if (r0 == 2) team_stage<4, E, NT, DIR>(buf, M, logTw, 1, tw, nst == 0 ? post : nullptr);
+  int Ns = 1 << r0;
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    if (s < nst) {
+      team_stage8<E, NT, DIR>(buf, M, logTw, Ns, T.w[s], pm, use_pm && s == nst - 1);
+      Ns *= 8;
+    }
+  }
+}
+
+// Middle of a length-n transform on the workgroup engine.  Direct power of
+// two: the FFT itself.  Bluestein (M >= 2n-1): the caller has loaded
+// x_k * chirp_k^{DIR} (k < n, zero up to M); this runs the forward FFT with
+// the filter folded into its last stage and the backward FFT (same
+// twiddles); the caller multiplies output k < n by chirp_k^{DIR}.
+template <int E, int NT, int DIR>
+__device__ __forceinline__ void team_fft_core(const FftSpec &s, double2 *buf, int logTw) {
+  TeamTw<E> T;
+  team_twiddles<E, NT>(T, s.M, logTw, s.tw);
+  double2 pm[E / 8][8];
+  if (!s.bluestein) {
+    team_fft<E, NT, DIR>(buf, s.M, logTw, T, s.tw, nullptr, pm, false);
+    return;
+  }
+  const double2 *F = DIR > 0 ? s.filt_b : s.filt_f;
+  const int M = s.M, nst = ilog2_dev(M) / 3;
+  if (nst > 0) {
+    // last radix-8 stage: Ns = M/8, output k = jj + r*M/8
+#pragma unroll
+    for (int q = 0; q < E / 8; ++q) {
+      const int jj = (threadIdx.x + NT * q) >> logTw;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) pm[q][r] = F[jj + r * (M >> 3)];
+    }
+  }
+  team_fft<E, NT, -1>(buf, M, logTw, T, s.tw, F, pm, nst > 0);
+  team_fft<E, NT, +1>(buf, M, logTw, T, s.tw, nullptr, pm, false);
+}
+
+// Workgroup id remapped so that consecutive logical tiles run on the same
+// XCD (the hardware deals workgroups to the 8 XCDs round-robin), which keeps
+// tiles that share cache lines in one L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  constexpr int X = 8;
+  const int q = nb / X, rm = nb % X;
+  const int x = bid % X, i = bid / X;
+  return x * q + (x < rm ? x : rm) + i;
+}
+
 }  // namespace s2dw

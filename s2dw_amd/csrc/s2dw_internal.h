@@ -135,6 +135,7 @@ struct Plan {
   int syn_q = 4;              // chains per lane in the synthesis contraction
   int ana_p = 2;              // chains per lane in the VALU analysis contraction
   bool ana_mfma = true;       // analysis contraction on FP64 MFMA
+  int fft_engine = 0;         // 0 workgroup engine, 1 wave engine, 2 generic Stockham
 };
 
 int plan_build(Plan *p, int B, int N, int J, double alpha, const double *K, const double *Phi2, const double *S,
