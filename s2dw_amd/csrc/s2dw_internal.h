@@ -132,7 +132,17 @@ struct Plan {
   bool timing = false;
   std::vector<StageEvents> ev_live;  // recorded, not yet read
   std::vector<hipEvent_t> ev_pool;   // free events
-  int syn_q = 4;              // chains per lane in the synthesis contraction
+  int syn_q = 4;              // chains per lane in the VALU synthesis contraction
+  bool syn_mfma = true;       // synthesis contraction on FP64 MFMA
+  int smf_NCp = 0, smf_CW = 0, smf_nch = 0, n_swork = 0;
+  int smf_wcount[3] = {0, 0, 0};  // work items per k-step class (<=2, <=4, <=8)
+  std::vector<int> smf_lb, smf_ub;
+  std::vector<long long> smf_btoff;
+  double *d_BT = nullptr;     // per-scale synthesis B tables
+  double2 *d_cc = nullptr;    // m'-recursion coefficients [m'][l]
+  double *d_cp = nullptr;     // checkpoint planes [chunk][2][m][l]
+  double2 *d_fpart = nullptr; // chunk partial sums [chunk][m][l]
+  int4 *d_swork = nullptr;    // MFMA synthesis work list
   int ana_p = 2;              // chains per lane in the VALU analysis contraction
   bool ana_mfma = true;       // analysis contraction on FP64 MFMA
   int fft_engine = 0;         // 0 workgroup engine, 1 wave engine, 2 generic Stockham
