@@ -7,6 +7,18 @@ HBM before the timed region.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|split]
                     [--no-cpu-baseline]
+
+The timed region runs the steps back to back (value, per-transform latency).
+A second pass over the same steps records HIP events around every stage
+launch on the plan's stream; the dominant stage's average launch time gives
+the roofline figures.  HBM traffic per launch comes from rocprofv3 counter
+runs, reduced by
+
+    python bench.py --traffic-from FETCH_DIR WRITE_DIR
+
+into profiles/traffic.json (FETCH_SIZE / WRITE_SIZE per dispatch, scaled by
+the same counters read on a plain device copy of known size, which
+``python bench.py --calib-copy`` runs).
 """
 from __future__ import annotations
 
@@ -87,6 +99,89 @@ def cpu_baseline(K, Phi2, S, J, bl, flm):
             "analysis_s": t1 - t0, "synthesis_s": t2 - t1}
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
+CALIB_BYTES = 256 << 20  # bytes read and written by the calibration copy
+CALIB_REPS = 4
+
+
+def calib_copy():
+    """A plain device-to-device copy of known size, for scaling the HBM
+    byte counters (run under rocprofv3 --pmc)."""
+    import torch
+
+    n = CALIB_BYTES // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(CALIB_REPS):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_bytes": CALIB_BYTES, "reps": CALIB_REPS}))
+
+
+def _counter_rows(path):
+    """All rows of every *counter_collection.csv under path (or of path)."""
+    import csv
+
+    files = [path]
+    if os.path.isdir(path):
+        files = [os.path.join(r, f) for r, _, fs in os.walk(path) for f in fs if f.endswith("counter_collection.csv")]
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def traffic_from(fetch_path, write_path):
+    """Per-dispatch HBM bytes of every kernel from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE, in KiB), each scaled by its reading on the
+    calibration copy (the largest copyBuffer dispatch of the run)."""
+    out = {}
+    for counter, path in (("FETCH_SIZE", fetch_path), ("WRITE_SIZE", write_path)):
+        rows = [r for r in _counter_rows(path) if r["Counter_Name"] == counter]
+        per = {}
+        for r in rows:
+            per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024.0)
+        copies = [v for name, vals in per.items() if "copyBuffer" in name for v in vals]
+        scale = CALIB_BYTES / max(copies) if copies else 1.0
+        for name, vals in per.items():
+            if "s2dw" not in name:
+                continue
+            key = name.split("(")[0].replace("void ", "").replace("s2dw::", "")
+            d = out.setdefault(key, {})
+            d[counter.lower() + "_bytes"] = scale * sum(vals) / len(vals)
+            d[counter.lower() + "_scale"] = scale
+            d["dispatches"] = len(vals)
+    for d in out.values():
+        d["hbm_bytes"] = d.get("fetch_size_bytes", 0.0) + d.get("write_size_bytes", 0.0)
+    os.makedirs(os.path.dirname(TRAFFIC_JSON), exist_ok=True)
+    with open(TRAFFIC_JSON, "w") as fh:
+        json.dump({"source": "rocprofv3 --pmc FETCH_SIZE, then --pmc WRITE_SIZE, each over "
+                             "'python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline' and "
+                             "'python3 bench.py --calib-copy'; per-dispatch KiB scaled by the counter's reading "
+                             f"on the {CALIB_BYTES >> 20} MiB calibration copy", "kernels": out}, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+# stage name -> kernel whose launches dominate it
+STAGE_KERNEL = {"ana_contract": "k_ana_contract_mfma", "syn_contract": "k_syn_contract_mfma",
+                "ana_beta_fft": "k_beta_ana_t", "ana_alpha_c2r": "k_alpha_c2r_t",
+                "syn_alpha_r2c": "k_alpha_r2c_t", "syn_beta_fft": "k_beta_syn_t"}
+
+
+def load_traffic(stage):
+    try:
+        with open(TRAFFIC_JSON) as fh:
+            data = json.load(fh)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    kern = STAGE_KERNEL.get(stage)
+    hits = [v for k, v in data.items() if kern and k.startswith(kern)]
+    if not hits:
+        return None
+    return sum(h["hbm_bytes"] * h["dispatches"] for h in hits) / sum(h["dispatches"] for h in hits)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,7 +189,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true", help="run the byte-counter calibration copy and exit")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH", "WRITE"),
+                    help="reduce two rocprofv3 counter runs into profiles/traffic.json and exit")
     args = ap.parse_args()
+    if args.traffic_from:
+        traffic_from(*args.traffic_from)
+        return
+    if args.calib_copy:
+        calib_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(requests shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
@@ -123,7 +227,6 @@ def main():
 
     steps = args.steps
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-    plan.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
@@ -134,8 +237,14 @@ def main():
         ev[i][2].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    plan.set_timing(False)
     wall = t1 - t0
+    # second pass: HIP events around every stage launch, on the plan's stream
+    plan.set_timing(True)
+    for i in range(steps):
+        plan.analysis(d_f, d_w, d_sc)
+        plan.synthesis(d_w, d_sc, d_g)
+    torch.cuda.synchronize()
+    plan.set_timing(False)
     st = plan.stage_times()
     lat_a = [ev[i][0].elapsed_time(ev[i][1]) for i in range(steps)]
     lat_s = [ev[i][1].elapsed_time(ev[i][2]) for i in range(steps)]
@@ -151,23 +260,27 @@ def main():
     for name, (ms, cnt) in st.items():
         if cnt:
             stages[name] = {"ms_per_step": ms / steps, "launches_per_step": cnt / steps}
-    # dominant kernel and its roofline
+    # dominant stage (largest time per step) and the roofline of its kernel:
+    # algorithmic work per launch over the average launch time from the events
     dom = max(stages, key=lambda k: stages[k]["ms_per_step"])
-    dom_ms = stages[dom]["ms_per_step"]
+    nl = max(1.0, stages[dom]["launches_per_step"])
+    launch_ms = stages[dom]["ms_per_step"] / nl
+    traffic = load_traffic(dom)
     if dom in ("ana_contract", "syn_contract"):
         cm = ana_cmac if dom == "ana_contract" else syn_cmac
-        work = 8.0 * cm  # 8 real flops per useful complex MAC
-        achieved = work / (dom_ms * 1e-3) / 1e12
-        roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                "work_per_launch": work / max(1, stages[dom]["launches_per_step"]),
-                "work_unit": "flop (8 per useful complex MAC)"}
+        work = 8.0 * cm / nl  # 8 real flops per useful complex MAC
+        achieved = work / (launch_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                "kernel": STAGE_KERNEL.get(dom, dom), "stage": dom, "avg_launch_ms": launch_ms,
+                "work_per_launch": work, "work_unit": "flop (8 per useful complex MAC)"}
     else:
-        byts = sb.get(dom, 0)
-        achieved = byts / (dom_ms * 1e-3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "work_per_launch": byts / max(1, stages[dom]["launches_per_step"]), "work_unit": "byte"}
+        work = sb.get(dom, 0) / nl
+        achieved = work / (launch_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": STAGE_KERNEL.get(dom, dom), "stage": dom, "avg_launch_ms": launch_ms,
+                "work_per_launch": work, "work_unit": "byte (compulsory)"}
     contraction = {}
     for name, cm in (("ana_contract", ana_cmac), ("syn_contract", syn_cmac)):
         if name in stages:
@@ -202,6 +315,7 @@ def main():
                        "synthesis_p99": float(np.percentile(lat_s, 99)),
                        "p50": float(np.percentile(lat_all, 50)), "p99": float(np.percentile(lat_all, 99))},
         "round_trip_max_err": rt_err,
+        "plan_device_bytes": plan.device_bytes,
         "stages": stages,
         "contraction": contraction,
         "fft_stages": ffts,

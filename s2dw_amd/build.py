@@ -21,8 +21,12 @@ ARCH = os.environ.get("S2DW_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no packed FP32 instructions anywhere in the device code.
+# amdgpu-mfma-vgpr-form: MFMA accumulators in VGPRs (gfx950 allows it), which
+# avoids AGPR<->VGPR copies around the MFMAs of loops whose accumulators are
+# also touched by VALU code.
 CFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-fopenmp",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-mllvm", "-amdgpu-mfma-vgpr-form=1",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-I" + os.path.join(HERE, "..", "include")]
 
 

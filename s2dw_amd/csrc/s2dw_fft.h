@@ -362,6 +362,15 @@ __device__ __forceinline__ void wave_fft_exec(const FftSpec &s, double2 *buf, in
 // the others are products of those (at most three roundings).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+
+// Workgroup barrier that orders LDS only: the fences are restricted to the
+// local address space, so outstanding global loads (prefetches) stay in
+// flight across it.  Used where the barrier protects LDS data alone.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 __host__ __device__ constexpr int lpad_size(int s) { return s + (s >> 4); }
 
 template <int R, int DIR>
@@ -420,7 +429,7 @@ __device__ __forceinline__ void team_stage(double2 *buf, int M, int logTw, int N
       for (int r = 0; r < R; ++r) pm[q][r] = post[idxD[q] + r * Ns];
     }
   }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     if (Ns > 1) twiddle_powers<R, DIR>(v[q], w1[q], w2[q], w4[q], w8[q]);
@@ -432,15 +441,15 @@ __device__ __forceinline__ void team_stage(double2 *buf, int M, int logTw, int N
       buf[lpad(((idxD[q] + r * Ns) << logTw) + tt[q])] = y;
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // Radix-8 stage with its twiddles (w, w^2, w^4 per butterfly) and, for the
 // last stage of a Bluestein forward transform, the filter values already in
 // registers.
-template <int E, int NT, int DIR>
+template <int E, int NT, int DIR, bool PM>
 __device__ __forceinline__ void team_stage8(double2 *buf, int M, int logTw, int Ns, const double2 (&w)[E / 8][3],
-                                            const double2 (&pm)[E / 8][8], bool use_pm) {
+                                            const double2 (&pm)[E / 8][8]) {
   constexpr int R = 8, NB = E / R;
   const int Tw = 1 << logTw;
   const int nR = M / R;
@@ -456,19 +465,20 @@ __device__ __forceinline__ void team_stage8(double2 *buf, int M, int logTw, int 
 #pragma unroll
     for (int r = 0; r < R; ++r) v[q][r] = buf[lpad(((jj + r * nR) << logTw) + tt[q])];
   }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
-    if (Ns > 1) twiddle_powers<R, DIR>(v[q], w[q][0], w[q][1], w[q][2], w[q][2]);
+    // for Ns = 1 the twiddles are exactly 1 and the products exact
+    twiddle_powers<R, DIR>(v[q], w[q][0], w[q][1], w[q][2], w[q][2]);
     dft_pow2<R, DIR>(v[q]);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       double2 y = v[q][r];
-      if (use_pm) y = cmul(y, pm[q][r]);
+      if constexpr (PM) y = cmul(y, pm[q][r]);
       buf[lpad(((idxD[q] + r * Ns) << logTw) + tt[q])] = y;
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // Power-of-two transform of all Tw sequences in the workgroup's slots, in
@@ -515,7 +525,8 @@ __device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const T
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     if (s < nst) {
-      team_stage8<E, NT, DIR>(buf, M, logTw, Ns, T.w[s], pm, use_pm && s == nst - 1);
+      if (use_pm && s == nst - 1) team_stage8<E, NT, DIR, true>(buf, M, logTw, Ns, T.w[s], pm);
+      else team_stage8<E, NT, DIR, false>(buf, M, logTw, Ns, T.w[s], pm);
       Ns *= 8;
     }
   }

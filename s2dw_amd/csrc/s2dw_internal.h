@@ -98,13 +98,16 @@ struct Plan {
   std::vector<double> K, Phi2, S;  // host copies (reference layouts)
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  hipStream_t aux = nullptr;          // second stream for small concurrent launches
+  hipEvent_t fork = nullptr, join = nullptr;
   // device tables
   double *d_S0 = nullptr;     // chain start values, triangle [l0][k]
   double *d_thin = nullptr;   // Delta^l_{m'k}, [tri(m',l)][k]
   double *d_Ba = nullptr;     // analysis rows [tri(m',l)][2 + 2*NCa]
   double *d_Bs = nullptr;     // synthesis rows [tri(l,m')][2 + 2*NCs]
-  int *d_lmeta = nullptr;     // per l: flush info (int2), then next flush l (int)
-  int *d_nextev = nullptr;    // inside d_lmeta
+  int *d_lmeta = nullptr;     // per l: flush info (int2), then per group of 4 l: end bits
+  int *d_gev = nullptr;       // inside d_lmeta
+  double *d_lcoef = nullptr;  // l-recursion coefficients [m'][l] (A, Bc)
   int4 *d_sblk = nullptr;     // synthesis l-blocks (l_first, count, slot-0 scale, slot-1 scale)
   int n_sblk = 0;
   double2 *d_G = nullptr;     // analysis gamma matrix [g][c]
