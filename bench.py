@@ -163,6 +163,122 @@ def traffic_from(fetch_path, write_path):
     print(json.dumps(out, indent=1))
 
 
+def _mix_requests(rng, count, Ls, Ns, alphas, log_uniform=False):
+    """Seeded request shapes (B, N, alpha, J=J_max) and f_lm."""
+    from s2dw_amd import core
+
+    reqs = []
+    for _ in range(count):
+        if log_uniform:
+            B = int(round(float(np.exp(rng.uniform(np.log(Ls[0]), np.log(Ls[1]))))))
+        else:
+            B = int(rng.choice(Ls))
+        N = int(rng.choice(Ns)) if not log_uniform else int(rng.integers(Ns[0], Ns[1] + 1))
+        N = min(N, B)
+        alpha = float(rng.choice(alphas))
+        J = core.s2dw_core_comp_Jmax(B, alpha)
+        reqs.append((B, N, alpha, J, seeded_flm(B, int(rng.integers(1 << 30)))))
+    return reqs
+
+
+def bench_batch(args):
+    """64 transforms (32 analyses, 32 syntheses of their resident outputs),
+    B from {64,128,256,512}, N from {1,3,5,8}, alpha = 2, J = J_max, all
+    submitted at once to the request loop; plans warmed by a first pass."""
+    from s2dw_amd.serve import Server
+
+    rng = np.random.default_rng(4)
+    reqs = _mix_requests(rng, 32, [64, 128, 256, 512], [1, 3, 5, 8], [2.0])
+    with Server(200 << 30, 16) as srv:
+        def one_pass():
+            t0 = time.perf_counter()
+            ta = [srv.submit_analysis(B, N, J, a, f, keep_resident=True, want_wav=False) for (B, N, a, J, f) in reqs]
+            ts = [srv.submit_synthesis(B, N, J, a, wav_from=t) for (B, N, a, J, f), t in zip(reqs, ta)]
+            for t in ta + ts:
+                srv.wait(t)
+            t1 = time.perf_counter()
+            err = max(float(np.abs(srv.result(t) - r[4]).max()) for t, r in zip(ts, reqs))
+            for t in ta:
+                srv.release(t)
+            return t1 - t0, err, [t.latency_ms for t in ta + ts]
+        one_pass()  # builds the plans
+        wall, err, lat = one_pass()
+    out = {"workload": "batch of 64 transforms (32 analysis + 32 synthesis), L in {64,128,256,512}, "
+                       "N in {1,3,5,8}, alpha=2, J=J_max, seeded f_lm",
+           "transforms_per_s": 64 / wall, "wall_ms": 1e3 * wall, "max_round_trip_err": err,
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+           "dtype": "f64", "n_gpus": 1}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Request loop under Poisson arrivals: B log-uniform in [32, 1024]
+    (rounded), N uniform in 1..8, alpha in {2, 3}, J = J_max; every request
+    is an analysis whose coefficients stay resident, half of them followed by
+    a synthesis of those coefficients.  A first pass builds the plans; the
+    second replays the arrivals in real time."""
+    import threading
+
+    from s2dw_amd.serve import Server
+
+    rng = np.random.default_rng(5)
+    n_req, rate = args.serve_requests, args.serve_rate
+    reqs = _mix_requests(rng, n_req, [32, 1024], [1, 8], [2.0, 3.0], log_uniform=True)
+    follow = rng.random(n_req) < 0.5
+    gaps = rng.exponential(1.0 / rate, n_req)
+    with Server(200 << 30, 16) as srv:
+        def one_pass(realtime):
+            tickets, pairs, lock = [], [], threading.Lock()
+            total = n_req + int(follow.sum())
+
+            def waiter():
+                i = 0
+                while i < total:
+                    with lock:
+                        t = tickets[i] if i < len(tickets) else None
+                    if t is None:
+                        time.sleep(1e-4)
+                        continue
+                    srv.wait(t)
+                    i += 1
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            t0 = time.perf_counter()
+            tnext = t0
+            for i, (B, N, a, J, f) in enumerate(reqs):
+                if realtime:
+                    tnext += gaps[i]
+                    while time.perf_counter() < tnext:
+                        time.sleep(max(0.0, min(1e-3, tnext - time.perf_counter())))
+                ta = srv.submit_analysis(B, N, J, a, f, keep_resident=True, want_wav=False)
+                with lock:
+                    tickets.append(ta)
+                if follow[i]:
+                    ts = srv.submit_synthesis(B, N, J, a, wav_from=ta)
+                    pairs.append((ts, f))
+                    with lock:
+                        tickets.append(ts)
+            th.join()
+            t1 = time.perf_counter()
+            for t in tickets:
+                if t.kind == "analysis":
+                    srv.release(t)
+            err = max(float(np.abs(ts.out - f).max()) for ts, f in pairs) if pairs else 0.0
+            return t1 - t0, [t.latency_ms for t in tickets], tickets, err
+        one_pass(False)
+        wall, lat, tickets, err = one_pass(True)
+    ntr = len(tickets)
+    out = {"workload": f"serving loop: {n_req} requests, Poisson arrivals at {rate}/s, B log-uniform in [32,1024], "
+                       "N uniform in 1..8, alpha in {2,3}, J=J_max; each an analysis kept resident, half followed "
+                       "by a synthesis of it",
+           "transforms": ntr, "transforms_per_s": ntr / wall, "wall_s": wall,
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(np.max(lat))},
+           "max_round_trip_err": err, "dtype": "f64", "n_gpus": 1}
+    print(json.dumps(out), flush=True)
+
+
 # stage name -> kernel whose launches dominate it
 STAGE_KERNEL = {"ana_contract": "k_ana_contract_mfma", "syn_contract": "k_syn_contract_mfma",
                 "ana_beta_fft": "k_beta_ana_t", "ana_alpha_c2r": "k_alpha_c2r_t",
@@ -192,6 +308,11 @@ def main():
     ap.add_argument("--calib-copy", action="store_true", help="run the byte-counter calibration copy and exit")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH", "WRITE"),
                     help="reduce two rocprofv3 counter runs into profiles/traffic.json and exit")
+    ap.add_argument("--workload", default="c3", choices=["c3", "batch", "serve"],
+                    help="c3: the L=512 N=8 round trip (default); batch: 64 mixed transforms; "
+                         "serve: the request loop under Poisson arrivals")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=400.0, help="requests per second")
     args = ap.parse_args()
     if args.traffic_from:
         traffic_from(*args.traffic_from)
@@ -208,6 +329,12 @@ def main():
 
     from s2dw_amd import core
 
+    if args.workload == "batch":
+        bench_batch(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
+        return
     dev = torch.device("cuda:0")
     J = core.s2dw_core_comp_Jmax(L, ALPHA) - J_MIN
     K, Phi2, bl = core.s2dw_core_init_kernels(J, L, ALPHA)

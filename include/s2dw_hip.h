@@ -176,6 +176,49 @@ int s2dw_hip_plan_set_timing(s2dw_hip_plan_t plan, int enable);
 int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count);
 
 /* ---------------------------------------------------------------------
+ * Request loop (no counterpart in the reference, whose routines run one call
+ * at a time from one host thread, s2dw_core_mod.f90).  A server owns a worker
+ * thread, a pool of plans and an HBM budget.  Requests of any (B, N, J,
+ * alpha) are queued FIFO and admitted while an execution slot is free and
+ * their device footprint (a plan of their shape unless an idle one exists,
+ * plus their buffers) fits the budget; idle plans are evicted LRU to make
+ * room.  Admitted requests run concurrently, each on its plan's stream.
+ * Results equal those of the single-call entry points bit for bit.
+ * Host arrays use the layouts of the dynamic entry points: flm[l + B*m],
+ * wavelet coefficients packed per scale (jj = 0..J, each (2hi-1)*2hi*N,
+ * alpha fastest), scoeff[l + bl*m].  Host arrays must stay valid until
+ * s2dw_hip_server_wait returns for the request.
+ * ------------------------------------------------------------------- */
+typedef struct s2dw_hip_server *s2dw_hip_server_t;
+
+int s2dw_hip_server_create(s2dw_hip_server_t *server, long long hbm_budget_bytes, int max_inflight);
+/* Finishes queued and running requests, then frees everything. */
+int s2dw_hip_server_destroy(s2dw_hip_server_t server);
+
+/* Analysis of flm.  wav (may be NULL) receives the packed coefficients,
+ * scoeff the scaling coefficients (may be NULL only with keep_resident).
+ * keep_resident != 0 keeps wav and scoeff on the device for a later
+ * synthesis (wav_from = this ticket) until s2dw_hip_server_release. */
+int s2dw_hip_server_submit_analysis(s2dw_hip_server_t server, int B, int N, int J, double alpha, const double *flm,
+                                    double *wav, double *scoeff, int keep_resident, long long *ticket);
+
+/* Synthesis into flm, from the resident result of analysis ticket wav_from
+ * (>= 0; it may still be queued or running) or from host wav and scoeff. */
+int s2dw_hip_server_submit_synthesis(s2dw_hip_server_t server, int B, int N, int J, double alpha, long long wav_from,
+                                     const double *wav, const double *scoeff, double *flm, long long *ticket);
+
+/* Blocks until the request has finished; returns its status.  latency_ms:
+ * submit to completion; run_ms: admission to completion (either may be NULL). */
+int s2dw_hip_server_wait(s2dw_hip_server_t server, long long ticket, double *latency_ms, double *run_ms);
+
+/* Frees the resident result of an analysis ticket (refused while a queued or
+ * running synthesis reads it). */
+int s2dw_hip_server_release(s2dw_hip_server_t server, long long ticket);
+
+int s2dw_hip_server_stats(s2dw_hip_server_t server, long long *in_use_bytes, int *inflight, int *queued,
+                          long long *completed);
+
+/* ---------------------------------------------------------------------
  * Diagnostics used by the tests: Wigner values Delta^l_{m'm}(pi/2) for
  * 0 <= m', m <= l generated by the device recursions of the transform
  * kernels.  out[m' + (l+1)*m] for the analysis (l-recursion) generator,

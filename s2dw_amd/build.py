@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libs2dw_hip.so")
-SOURCES = ["s2dw_setup.cpp", "s2dw_kernels.hip", "s2dw_capi.cpp"]
+SOURCES = ["s2dw_setup.cpp", "s2dw_kernels.hip", "s2dw_capi.cpp", "s2dw_serve.cpp"]
 ARCH = os.environ.get("S2DW_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
