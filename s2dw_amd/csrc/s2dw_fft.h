@@ -492,10 +492,13 @@ struct TeamTw {
 
 __device__ __forceinline__ int ilog2_dev(int x) { return 31 - __clz(x); }
 
+// ns0: Ns of the first radix-8 stage (1 << (log2 M mod 3) when the radix-2/4
+// stage comes first, 1 when it comes last)
 template <int E, int NT>
-__device__ __forceinline__ void team_twiddles(TeamTw<E> &T, int M, int logTw, const double2 *__restrict__ tw) {
+__device__ __forceinline__ void team_twiddles(TeamTw<E> &T, int M, int logTw, const double2 *__restrict__ tw,
+                                              int ns0 = -1) {
   const int logM = ilog2_dev(M), nst = logM / 3;
-  int Ns = 1 << (logM % 3);
+  int Ns = ns0 > 0 ? ns0 : 1 << (logM % 3);
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     if (s < nst) {
@@ -532,13 +535,116 @@ __device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const T
   }
 }
 
+// Bluestein with the forward transform's last stage handed to the backward
+// transform's first stage in registers.  Forward: radix R0 = 2^(log2 M mod 3)
+// first, then radix-8 stages; its last stage (Ns = M/8) leaves outputs
+// jj + r*M/8 of butterfly jj in registers, multiplied by the filter.  The
+// backward transform runs its radix-8 stages first: the first (Ns = 1)
+// reads exactly those elements of butterfly jj, so it starts from the
+// registers; its radix-R0 stage comes last (Ns = M/R0) and multiplies its
+// outputs by post_last[k] (k < M) if given.  Saves one LDS write, one LDS
+// read and a barrier pair per transform against two separate FFTs.
+template <int E, int NT, int DIR>
+__device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, int logTw,
+                                               const double2 *__restrict__ post_last) {
+  constexpr int NB = E / 8;
+  const int M = s.M, logM = ilog2_dev(M), nst = logM / 3, r0 = logM % 3;
+  const double2 *F = DIR > 0 ? s.filt_b : s.filt_f;
+  const int Tw = 1 << logTw, nR = M >> 3;
+  TeamTw<E> T;
+  team_twiddles<E, NT>(T, M, logTw, s.tw);
+  double2 pm[NB][8];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int jj = (threadIdx.x + NT * q) >> logTw;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) pm[q][r] = F[jj + r * nR];
+  }
+  // forward
+  if (r0 == 1) team_stage<2, E, NT, -1>(buf, M, logTw, 1, s.tw, nullptr);
+  else if (r0 == 2) team_stage<4, E, NT, -1>(buf, M, logTw, 1, s.tw, nullptr);
+  int Ns = 1 << r0;
+#pragma unroll
+  for (int st = 0; st < 2; ++st) {
+    if (st < nst - 1) {
+      team_stage8<E, NT, -1, false>(buf, M, logTw, Ns, T.w[st], pm);
+      Ns *= 8;
+    }
+  }
+  double2 v[NB][8];
+  int tt[NB], jjv[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int b = threadIdx.x + NT * q;
+    tt[q] = b & (Tw - 1);
+    jjv[q] = b >> logTw;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[q][r] = buf[lpad(((jjv[q] + r * nR) << logTw) + tt[q])];
+  }
+  // last forward stage's twiddles, then the backward transform's (8-first order)
+  double2 wl[NB][3];
+#pragma unroll
+  for (int q = 0; q < NB; ++q)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) wl[q][i] = T.w[2][q][i];
+  if (nst == 2) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) wl[q][i] = T.w[1][q][i];
+  } else if (nst == 1) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) wl[q][i] = T.w[0][q][i];
+  }
+  team_twiddles<E, NT>(T, M, logTw, s.tw, 1);
+  lds_barrier();  // every read of the forward transform is done
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    twiddle_powers<8, -1>(v[q], wl[q][0], wl[q][1], wl[q][2], wl[q][2]);
+    dft_pow2<8, -1>(v[q]);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[q][r] = cmul(v[q][r], pm[q][r]);
+    // backward first stage, Ns = 1: no twiddles, outputs 8 jj + r
+    dft_pow2<8, +1>(v[q]);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) buf[lpad(((jjv[q] * 8 + r) << logTw) + tt[q])] = v[q][r];
+  }
+  lds_barrier();
+  Ns = 8;
+#pragma unroll
+  for (int st = 1; st < 3; ++st) {
+    if (st < nst) {
+      team_stage8<E, NT, +1, false>(buf, M, logTw, Ns, T.w[st], pm);
+      Ns *= 8;
+    }
+  }
+  if (r0 == 1) team_stage<2, E, NT, +1>(buf, M, logTw, Ns, s.tw, post_last);
+  else if (r0 == 2) team_stage<4, E, NT, +1>(buf, M, logTw, Ns, s.tw, post_last);
+  else if (post_last) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int i = threadIdx.x + NT * q;
+      buf[lpad(i)] = cmul(buf[lpad(i)], post_last[i >> logTw]);
+    }
+    lds_barrier();
+  }
+}
+
 // Middle of a length-n transform on the workgroup engine.  Direct power of
 // two: the FFT itself.  Bluestein (M >= 2n-1): the caller has loaded
 // x_k * chirp_k^{DIR} (k < n, zero up to M); this runs the forward FFT with
-// the filter folded into its last stage and the backward FFT (same
-// twiddles); the caller multiplies output k < n by chirp_k^{DIR}.
+// the filter folded into its last stage and the backward FFT; the output k
+// is multiplied by post_last[k] if given (the caller multiplies by
+// chirp_k^{DIR} otherwise).
 template <int E, int NT, int DIR>
-__device__ __forceinline__ void team_fft_core(const FftSpec &s, double2 *buf, int logTw) {
+__device__ __forceinline__ void team_fft_core(const FftSpec &s, double2 *buf, int logTw,
+                                              const double2 *__restrict__ post_last = nullptr) {
+  if (s.bluestein && ilog2_dev(s.M) >= 3) {
+    team_bluestein<E, NT, DIR>(s, buf, logTw, post_last);
+    return;
+  }
   TeamTw<E> T;
   team_twiddles<E, NT>(T, s.M, logTw, s.tw);
   double2 pm[E / 8][8];
@@ -558,7 +664,15 @@ __device__ __forceinline__ void team_fft_core(const FftSpec &s, double2 *buf, in
     }
   }
   team_fft<E, NT, -1>(buf, M, logTw, T, s.tw, F, pm, nst > 0);
-  team_fft<E, NT, +1>(buf, M, logTw, T, s.tw, nullptr, pm, false);
+  team_fft<E, NT, +1>(buf, M, logTw, T, s.tw, post_last, pm, false);
+  if (post_last && nst > 0) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int i = threadIdx.x + NT * q;
+      buf[lpad(i)] = cmul(buf[lpad(i)], post_last[i >> logTw]);
+    }
+    lds_barrier();
+  }
 }
 
 // Workgroup id remapped so that consecutive logical tiles run on the same

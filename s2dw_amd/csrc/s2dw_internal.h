@@ -60,7 +60,8 @@ struct FftSpec {
   int nst;        // number of Stockham stages for M
   int radix[kMaxStages];
   const double2 *tw;     // M entries, e^{+2 pi i k / M}
-  const double2 *chirp;  // n entries, e^{+i pi (k^2 mod 2n) / n}       (Bluestein)
+  const double2 *chirp;  // M entries, e^{+i pi (k^2 mod 2n) / n} for k < n, 0 beyond (Bluestein)
+  const double2 *chirp_c; // M entries, conj(chirp)                         (Bluestein)
   const double2 *filt_b; // M entries, FFT_M(conj chirp) / M  (backward transforms)
   const double2 *filt_f; // M entries, FFT_M(chirp) / M       (forward transforms)
 };
