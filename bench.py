@@ -312,7 +312,7 @@ def main():
                     help="c3: the L=512 N=8 round trip (default); batch: 64 mixed transforms; "
                          "serve: the request loop under Poisson arrivals")
     ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--serve-rate", type=float, default=400.0, help="requests per second")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second")
     args = ap.parse_args()
     if args.traffic_from:
         traffic_from(*args.traffic_from)

@@ -545,10 +545,10 @@ __device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const T
 // outputs by post_last[k] (k < M) if given.  Saves one LDS write, one LDS
 // read and a barrier pair per transform against two separate FFTs.
 template <int E, int NT, int DIR>
-__device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, int logTw,
+__device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, int M, int logTw,
                                                const double2 *__restrict__ post_last) {
   constexpr int NB = E / 8;
-  const int M = s.M, logM = ilog2_dev(M), nst = logM / 3, r0 = logM % 3;
+  const int logM = ilog2_dev(M), nst = logM / 3, r0 = logM % 3;
   const double2 *F = DIR > 0 ? s.filt_b : s.filt_f;
   const int Tw = 1 << logTw, nR = M >> 3;
   TeamTw<E> T;
@@ -637,23 +637,25 @@ __device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, i
 // x_k * chirp_k^{DIR} (k < n, zero up to M); this runs the forward FFT with
 // the filter folded into its last stage and the backward FFT; the output k
 // is multiplied by post_last[k] if given (the caller multiplies by
-// chirp_k^{DIR} otherwise).
+// chirp_k^{DIR} otherwise).  M, logTw and bluestein are passed separately
+// from s so that kernels specialised on the work length hand down
+// compile-time constants (every function here is inlined).
 template <int E, int NT, int DIR>
-__device__ __forceinline__ void team_fft_core(const FftSpec &s, double2 *buf, int logTw,
+__device__ __forceinline__ void team_fft_core(const FftSpec &s, double2 *buf, int M, int logTw, bool bluestein,
                                               const double2 *__restrict__ post_last = nullptr) {
-  if (s.bluestein && ilog2_dev(s.M) >= 3) {
-    team_bluestein<E, NT, DIR>(s, buf, logTw, post_last);
+  if (bluestein && ilog2_dev(M) >= 3) {
+    team_bluestein<E, NT, DIR>(s, buf, M, logTw, post_last);
     return;
   }
   TeamTw<E> T;
-  team_twiddles<E, NT>(T, s.M, logTw, s.tw);
+  team_twiddles<E, NT>(T, M, logTw, s.tw);
   double2 pm[E / 8][8];
-  if (!s.bluestein) {
-    team_fft<E, NT, DIR>(buf, s.M, logTw, T, s.tw, nullptr, pm, false);
+  if (!bluestein) {
+    team_fft<E, NT, DIR>(buf, M, logTw, T, s.tw, nullptr, pm, false);
     return;
   }
   const double2 *F = DIR > 0 ? s.filt_b : s.filt_f;
-  const int M = s.M, nst = ilog2_dev(M) / 3;
+  const int nst = ilog2_dev(M) / 3;
   if (nst > 0) {
     // last radix-8 stage: Ns = M/8, output k = jj + r*M/8
 #pragma unroll
