@@ -171,6 +171,15 @@ void *s2dw_hip_plan_stream(s2dw_hip_plan_t plan);
  * s2dw_test.f90:97-135). */
 #define S2DW_NUM_STAGES 9
 int s2dw_hip_plan_set_timing(s2dw_hip_plan_t plan, int enable);
+/* Graph replay for s2dw_hip_analysis_device / s2dw_hip_synthesis_device
+ * (enable != 0): the first call with a given set of buffers captures the
+ * whole transform (every kernel of both streams) into a HIP graph, later
+ * calls with the same buffers replay it on the plan's stream.  Up to 8
+ * graphs per plan (least recently used dropped); disabling drops them all.
+ * Results are identical to eager launches.  Stage timing, while on, runs
+ * eagerly.  No counterpart in the reference (which has no launch cost to
+ * amortise). */
+int s2dw_hip_plan_set_graphs(s2dw_hip_plan_t plan, int enable);
 /* Sums (ms) and launch counts per stage of everything recorded since the
  * last call; waits for the recorded work.  ms and count hold 9 entries. */
 int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count);

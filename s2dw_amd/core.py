@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
         L.s2dw_hip_plan_stream.restype = C.c_void_p
         L.s2dw_hip_wigner_plane.argtypes = [C.c_int, _dp, _dp]
         L.s2dw_hip_plan_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.s2dw_hip_plan_set_graphs.argtypes = [C.c_void_p, C.c_int]
         L.s2dw_hip_plan_stage_times.argtypes = [C.c_void_p, _dp, _ip]
         _lib = L
     return _lib
@@ -325,6 +326,10 @@ class Plan:
 
     def set_timing(self, enable: bool):
         _check(lib().s2dw_hip_plan_set_timing(self.h, int(bool(enable))), "s2dw_hip_plan_set_timing")
+
+    def set_graphs(self, enable: bool):
+        """Replay each (direction, buffers) combination as a captured HIP graph."""
+        _check(lib().s2dw_hip_plan_set_graphs(self.h, int(bool(enable))), "s2dw_hip_plan_set_graphs")
 
     def stage_times(self):
         """{stage: (total ms, launches)} for everything recorded since the last call."""

@@ -305,12 +305,12 @@ long long s2dw_hip_plan_device_bytes(s2dw_hip_plan_t plan) { return plan ? plan-
 
 int s2dw_hip_analysis_device(s2dw_hip_plan_t plan, double *d_wav, double *d_scoeff, const double *d_flm) {
   if (!plan || !d_wav || !d_scoeff || !d_flm) return kErrArgInvalid;
-  return run_analysis(&plan->p, d_wav, reinterpret_cast<double2 *>(d_scoeff), reinterpret_cast<const double2 *>(d_flm));
+  return run_graphed(&plan->p, 0, d_wav, d_scoeff, d_flm);
 }
 
 int s2dw_hip_synthesis_device(s2dw_hip_plan_t plan, double *d_flm, const double *d_wav, const double *d_scoeff) {
   if (!plan || !d_wav || !d_scoeff || !d_flm) return kErrArgInvalid;
-  return run_synthesis(&plan->p, reinterpret_cast<double2 *>(d_flm), d_wav, reinterpret_cast<const double2 *>(d_scoeff));
+  return run_graphed(&plan->p, 1, d_flm, const_cast<double *>(d_wav), d_scoeff);
 }
 
 int s2dw_hip_plan_sync(s2dw_hip_plan_t plan) {
@@ -326,6 +326,13 @@ int s2dw_hip_wigner_plane(int l, double *out, double *out2) { return wigner_plan
 int s2dw_hip_plan_set_timing(s2dw_hip_plan_t plan, int enable) {
   if (!plan) return kErrArgInvalid;
   plan->p.timing = enable != 0;
+  return kErrNone;
+}
+
+int s2dw_hip_plan_set_graphs(s2dw_hip_plan_t plan, int enable) {
+  if (!plan) return kErrArgInvalid;
+  plan->p.graphs = enable != 0;
+  if (!plan->p.graphs) plan_drop_graphs(&plan->p);
   return kErrNone;
 }
 

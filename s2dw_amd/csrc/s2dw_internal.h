@@ -150,6 +150,20 @@ struct Plan {
   int ana_p = 2;              // chains per lane in the VALU analysis contraction
   bool ana_mfma = true;       // analysis contraction on FP64 MFMA
   int fft_engine = 0;         // 0 workgroup engine, 1 wave engine, 2 generic Stockham
+  // HIP graphs of whole transforms for the device API: one executable graph
+  // per (direction, buffers), captured on `cap` at first use and replayed
+  // on `stream` afterwards
+  struct GraphEntry {
+    int kind;
+    const void *b0, *b1, *b2;
+    hipGraphExec_t exec;
+    unsigned long long last;
+  };
+  void *groups = nullptr;  // grouped launches of the small scales (kernels' own state)
+  bool graphs = false;
+  hipStream_t cap = nullptr;
+  std::vector<GraphEntry> gcache;
+  unsigned long long gtick = 0;
 };
 
 int plan_build(Plan *p, int B, int N, int J, double alpha, const double *K, const double *Phi2, const double *S,
@@ -157,6 +171,10 @@ int plan_build(Plan *p, int B, int N, int J, double alpha, const double *K, cons
 void plan_free(Plan *p);
 int run_analysis(Plan *p, double *d_wav, double2 *d_scoeff, const double2 *d_flm);
 int run_synthesis(Plan *p, double2 *d_flm, const double *d_wav, const double2 *d_scoeff);
+// run_analysis (kind 0) or run_synthesis (kind 1) through the plan's graph
+// cache when graphs are on and stage timing is off, eagerly otherwise
+int run_graphed(Plan *p, int kind, void *out0, void *out1, const void *in);
+void plan_drop_graphs(Plan *p);
 int stage_times(Plan *p, double *ms, int *count);
 int pack_static(Plan *p, const double *d_packed, double *d_static, bool to_static);
 int wigner_plane(int l, double *out, double *out2);

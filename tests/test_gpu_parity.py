@@ -189,6 +189,51 @@ def test_device_plan_api_matches_host_api():
     plan.close()
 
 
+@pytest.mark.parametrize("B,N,stream", [(64, 5, "current"), (128, 3, "side")])
+def test_graph_replay_matches_eager(B, N, stream):
+    """Captured-graph replays, interleaved with eager work on the same plan and
+    host synchronisations, give the eager results bit for bit on every replay
+    (tolerance 0: the same kernels run on the same buffers)."""
+    import torch
+
+    alpha = 2.0
+    say(f"graph replay B={B} N={N} stream={stream}")
+    st = setup(B, N, alpha)
+    s = torch.cuda.current_stream() if stream == "current" else torch.cuda.Stream()
+    plan = core.Plan(B, N, st.J, alpha, st.K, st.Phi2, st.S, stream=s)
+    fs = [seeded_flm(B, 40 + i) for i in range(3)]
+    d_f = torch.from_numpy(np.ascontiguousarray(fs[0].T)).cuda()
+    d_w, d_sc, d_g = plan.empty_wav(), plan.empty_scoeff(), plan.empty_flm()
+    e_w, e_sc, e_g = plan.empty_wav(), plan.empty_scoeff(), plan.empty_flm()
+    ref = []
+    for f in fs:  # eager references
+        d_f.copy_(torch.from_numpy(np.ascontiguousarray(f.T)))
+        torch.cuda.synchronize()
+        plan.analysis(d_f, e_w, e_sc)
+        plan.synthesis(e_w, e_sc, e_g)
+        plan.sync()
+        ref.append((e_w.cpu(), e_sc.cpu(), e_g.cpu()))
+    plan.set_graphs(True)
+    for rep in range(2):
+        for i, f in enumerate(fs):
+            d_f.copy_(torch.from_numpy(np.ascontiguousarray(f.T)))
+            torch.cuda.synchronize()
+            plan.analysis(d_f, d_w, d_sc)   # captured at rep 0, i 0; replayed after
+            plan.synthesis(d_w, d_sc, d_g)
+            plan.sync()
+            torch.testing.assert_close(d_w.cpu(), ref[i][0], rtol=0, atol=0)
+            torch.testing.assert_close(d_sc.cpu(), ref[i][1], rtol=0, atol=0)
+            torch.testing.assert_close(d_g.cpu(), ref[i][2], rtol=0, atol=0)
+            # eager work on the same plan between replays
+            plan.set_timing(True)
+            plan.analysis(d_f, e_w, e_sc)
+            plan.set_timing(False)
+            plan.stage_times()
+            plan.sync()
+    plan.set_graphs(False)
+    plan.close()
+
+
 def test_invalid_bl_scoeff_raises():
     st = setup(32, 3, 2.0)
     f = seeded_flm(32, 1)
