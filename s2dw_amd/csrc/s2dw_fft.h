@@ -348,8 +348,12 @@ __device__ __forceinline__ void wave_fft_exec(const FftSpec &s, double2 *buf, in
 // Workgroup FFT engine.  A workgroup of NT threads owns S = NT*E complex LDS
 // slots holding Tw = S/M interleaved sequences of power-of-two length M
 // (logical slot k*Tw + t holds element k of sequence t).  The physical LDS
-// index of logical slot i is i + i/16: one pad slot per 16 spreads the
-// stride-R writes of the early Stockham stages over the LDS banks.
+// index of logical slot i is i ^ ((i >> 3) & 15), a permutation inside each
+// aligned block of 16 slots: 16 consecutive lanes that read consecutive
+// slots, and 16 consecutive lanes that write the slots of a radix-8 stage
+// (8*Tw apart at Ns = 1, interleaved groups at larger Ns), all fall in
+// distinct 16-byte bank groups.  Offsets that are multiples of 128 slots
+// pass through unchanged, so compile-time strides stay immediates.
 //
 // Stage (radix R, Ns = product of the radices before it): the q-th
 // butterfly b = threadIdx.x + NT*q, (jj, tt) = (b / Tw, b % Tw), reads
@@ -361,7 +365,7 @@ __device__ __forceinline__ void wave_fft_exec(const FftSpec &s, double2 *buf, in
 // from a table of e^{+2 pi i k/M} (k < M) as the powers w, w^2, w^4, w^8;
 // the others are products of those (at most three roundings).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__device__ __forceinline__ int lpad(int i) { return i ^ ((i >> 3) & 15); }
 
 // Workgroup barrier that orders LDS only: the fences are restricted to the
 // local address space, so outstanding global loads (prefetches) stay in
@@ -371,7 +375,7 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
-__host__ __device__ constexpr int lpad_size(int s) { return s + (s >> 4); }
+__host__ __device__ constexpr int lpad_size(int s) { return (s + 15) & ~15; }
 
 template <int R, int DIR>
 __device__ __forceinline__ void twiddle_powers(double2 *v, double2 w1, double2 w2, double2 w4, double2 w8) {
