@@ -105,6 +105,7 @@ struct Plan {
   double *d_S0 = nullptr;     // chain start values, triangle [l0][k]
   double *d_thin = nullptr;   // Delta^l_{m'k}, [tri(m',l)][k]
   double *d_Ba = nullptr;     // analysis rows [tri(m',l)][2 + 2*NCa]
+  long long ba_bytes = 0;     // its size in bytes
   double *d_Bs = nullptr;     // synthesis rows [tri(l,m')][2 + 2*NCs]
   int *d_lmeta = nullptr;     // per l: flush info (int2), then per group of 4 l: end bits
   int *d_gev = nullptr;       // inside d_lmeta
@@ -149,6 +150,7 @@ struct Plan {
   int4 *d_swork = nullptr;    // MFMA synthesis work list
   int ana_p = 2;              // chains per lane in the VALU analysis contraction
   bool ana_mfma = true;       // analysis contraction on FP64 MFMA
+  bool ana_wave_indep = true; // ... with independent waves (else workgroup-staged operands)
   int fft_engine = 0;         // 0 workgroup engine, 1 wave engine, 2 generic Stockham
   // HIP graphs of whole transforms for the device API: one executable graph
   // per (direction, buffers), captured on `cap` at first use and replayed
