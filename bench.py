@@ -280,7 +280,7 @@ def bench_serve(args):
 
 
 # stage name -> kernel whose launches dominate it
-STAGE_KERNEL = {"ana_contract": "k_ana_contract_mfma", "syn_contract": "k_syn_contract_mfma",
+STAGE_KERNEL = {"ana_contract": "k_ana_contract_mfma_w", "syn_contract": "k_syn_contract_mfma",
                 "ana_beta_fft": "k_beta_ana_t", "ana_alpha_c2r": "k_alpha_c2r_t",
                 "syn_alpha_r2c": "k_alpha_r2c_t", "syn_beta_fft": "k_beta_syn_t"}
 
