@@ -191,12 +191,15 @@ int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count);
  * alpha) are queued FIFO and admitted while an execution slot is free and
  * their device footprint (a plan of their shape unless an idle one exists,
  * plus their buffers) fits the budget; idle plans are evicted LRU to make
- * room.  Admitted requests run concurrently, each on its plan's stream.
- * Results equal those of the single-call entry points bit for bit.
- * Host arrays use the layouts of the dynamic entry points: flm[l + B*m],
- * wavelet coefficients packed per scale (jj = 0..J, each (2hi-1)*2hi*N,
- * alpha fastest), scoeff[l + bl*m].  Host arrays must stay valid until
- * s2dw_hip_server_wait returns for the request.
+ * room.  Up to min(max_inflight, 4) admitted requests run concurrently,
+ * each on a stream (lane) of the server's.  Device buffers are pooled by
+ * size; pooled bytes count against the budget and are freed first when room
+ * is needed.  Results equal those of the single-call entry points bit for
+ * bit.  Host arrays use the layouts of the dynamic entry points:
+ * flm[l + B*m], wavelet coefficients packed per scale (jj = 0..J, each
+ * (2hi-1)*2hi*N, alpha fastest), scoeff[l + bl*m].  Inputs are copied into
+ * pinned staging buffers by the submit call; outputs are copied out by
+ * s2dw_hip_server_wait, so output arrays must stay valid until it returns.
  * ------------------------------------------------------------------- */
 typedef struct s2dw_hip_server *s2dw_hip_server_t;
 

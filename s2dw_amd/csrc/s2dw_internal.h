@@ -100,6 +100,7 @@ struct Plan {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   hipStream_t aux = nullptr;          // second stream for small concurrent launches
+  bool single_stream = false;         // keep every launch on `stream` (callers that run many plans at once)
   hipEvent_t fork = nullptr, join = nullptr;
   // device tables
   double *d_S0 = nullptr;     // chain start values, triangle [l0][k]

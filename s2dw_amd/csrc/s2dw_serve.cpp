@@ -12,6 +12,18 @@
 // analysis's completion event, so it can be admitted while the analysis
 // still runs.  Results do not depend on what else is in flight: every
 // request runs the same kernels, in the same order, on buffers of its own.
+//
+// Requests run on a fixed set of lanes, one stream each, at most
+// kMaxLanes (the hardware queues a process gets by default): each request
+// keeps all its launches on its lane's stream, so concurrent requests never
+// share a hardware queue and a synthesis waiting for its analysis stalls
+// only its own lane.
+//
+// The worker never blocks on a transfer: host data goes through pinned
+// staging buffers that the client threads fill (submit) and drain (wait),
+// so every copy the worker issues is asynchronous.  Device and pinned
+// buffers are pooled by size; pooled device bytes count against the budget
+// and are the first thing freed when room is needed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,6 +57,33 @@ struct PlanEntry {
   Plan p;
   Key key{};
   Clock::time_point last_used;
+  hipStream_t own = nullptr;  // the plan's stream (its launches go to a lane's)
+};
+
+constexpr int kMaxLanes = 4;
+
+// Buffers cached by exact size.
+struct Pool {
+  std::multimap<size_t, void *> free;
+  long long bytes = 0;
+  void *take(size_t n) {
+    auto it = free.find(n);
+    if (it == free.end()) return nullptr;
+    void *q = it->second;
+    free.erase(it);
+    bytes -= (long long)n;
+    return q;
+  }
+  void put(void *q, size_t n) {
+    if (!q) return;
+    free.emplace(n, q);
+    bytes += (long long)n;
+  }
+};
+
+struct Buf {
+  void *ptr = nullptr;
+  size_t n = 0;
 };
 
 struct Request {
@@ -57,10 +96,13 @@ struct Request {
   double *out_host2 = nullptr;       // analysis: scoeff
   long long wav_from = -1;           // synthesis: ticket of the analysis whose wav to use
   bool keep_resident = false;        // analysis: keep wav + scoeff on the device
+  // pinned staging: inputs filled at submit, outputs drained at wait
+  Buf h_in, h_in2, h_out, h_out2;
   // execution state
   PlanEntry *plan = nullptr;
+  int lane = -1;
   void *d_in = nullptr, *d_out = nullptr, *d_sc = nullptr;
-  long long bytes = 0;  // device bytes held by this request's buffers
+  size_t n_in = 0, n_out = 0, n_sc = 0;
   hipEvent_t done = nullptr;
   int status = kErrNone;
   std::string error;  // detail text of a failure (raised on the worker thread)
@@ -70,7 +112,7 @@ struct Request {
 
 struct Resident {
   void *d_wav = nullptr, *d_sc = nullptr;
-  long long bytes = 0;
+  size_t n_wav = 0, n_sc = 0;
   hipEvent_t ready = nullptr;  // recorded by the producing analysis
   Key key{};
 };
@@ -92,7 +134,14 @@ struct s2dw_hip_server {
   std::map<long long, std::shared_ptr<s2dw::Request>> all;  // by ticket, until waited on
   std::map<long long, s2dw::Resident> resident;             // analysis outputs kept on the device
   std::multimap<s2dw::Key, s2dw::PlanEntry *> idle;         // idle plans by shape
+  std::map<s2dw::Key, int> nplans;                          // plans built, per shape (idle or busy)
   long long completed = 0;
+  std::vector<hipStream_t> lanes;  // one stream per concurrent request
+  std::vector<char> lane_busy;
+  s2dw::Pool dpool;               // device buffers (counted in in_use)
+  std::mutex pin_mu;              // guards hpool (client threads and the worker)
+  s2dw::Pool hpool;               // pinned host buffers
+  long long pin_cap = 8LL << 30;  // pinned bytes kept cached at most
 };
 
 namespace s2dw {
@@ -133,99 +182,228 @@ int build_plan(const Key &k, PlanEntry **out) {
     return err;
   }
   e->key = k;
+  e->own = e->p.stream;
+  e->p.single_stream = true;
   *out = e;
   return kErrNone;
 }
 
 void free_plan(s2dw_hip_server *s, PlanEntry *e) {
   s->in_use -= e->p.device_bytes;
+  --s->nplans[e->key];
+  e->p.stream = e->own;
   plan_free(&e->p);
   delete e;
 }
 
-// Device bytes a request needs besides its plan.
-long long request_bytes(const Request &r) {
-  const long long wav = wav_doubles_of(r.key) * 8, sc = 16LL * bl_of(r.key) * bl_of(r.key);
-  const long long flm = 16LL * r.key.B * r.key.B;
-  if (r.kind == 0) return flm + wav + sc;
-  return flm + (r.wav_from >= 0 ? 0 : wav + sc);
+// Byte sizes of a request's host-side arrays (and device buffers).
+struct Sizes {
+  size_t flm, wav, sc;
+};
+Sizes sizes_of(const Key &k) {
+  const int bl = bl_of(k);
+  return Sizes{(size_t)16 * k.B * k.B, (size_t)wav_doubles_of(k) * 8, (size_t)16 * bl * bl};
+}
+
+// Pinned host buffers, taken and returned by client threads and the worker.
+bool pin_get(s2dw_hip_server *s, Buf &b, size_t n) {
+  b = Buf{};
+  if (n == 0) return true;
+  {
+    std::lock_guard<std::mutex> g(s->pin_mu);
+    b.ptr = s->hpool.take(n);
+  }
+  if (!b.ptr && hipHostMalloc(&b.ptr, n, hipHostMallocPortable) != hipSuccess) {
+    b.ptr = nullptr;
+    return false;
+  }
+  b.n = n;
+  return true;
+}
+
+void pin_put(s2dw_hip_server *s, Buf &b) {
+  if (!b.ptr) return;
+  bool keep;
+  {
+    std::lock_guard<std::mutex> g(s->pin_mu);
+    keep = s->hpool.bytes + (long long)b.n <= s->pin_cap;
+    if (keep) s->hpool.put(b.ptr, b.n);
+  }
+  if (!keep) (void)hipHostFree(b.ptr);
+  b = Buf{};
+}
+
+void pin_release_all(s2dw_hip_server *s, Request &r) {
+  pin_put(s, r.h_in);
+  pin_put(s, r.h_in2);
+  pin_put(s, r.h_out);
+  pin_put(s, r.h_out2);
+}
+
+// Device buffers (worker, under the server lock): pooled bytes stay counted
+// in in_use until freed.
+void dev_put(s2dw_hip_server *s, void *q, size_t n) {
+  if (q) s->dpool.put(q, n);
+}
+
+// Frees pooled device buffers, then idle plans (least recently used first),
+// until need more bytes fit the budget.
+bool evict_for(s2dw_hip_server *s, long long need) {
+  while (s->in_use + need > s->budget && !s->dpool.free.empty()) {
+    auto big = std::prev(s->dpool.free.end());
+    (void)hipFree(big->second);
+    s->in_use -= (long long)big->first;
+    s->dpool.bytes -= (long long)big->first;
+    s->dpool.free.erase(big);
+  }
+  while (s->in_use + need > s->budget && !s->idle.empty()) {
+    auto lru = s->idle.begin();
+    for (auto jt = s->idle.begin(); jt != s->idle.end(); ++jt)
+      if (jt->second->last_used < lru->second->last_used) lru = jt;
+    PlanEntry *victim = lru->second;
+    s->idle.erase(lru);
+    free_plan(s, victim);
+  }
+  return s->in_use + need <= s->budget;
+}
+
+// Device buffer sizes a request needs (0 = none): input, output, scoeff.
+void dev_sizes(const Request &r, size_t &n_in, size_t &n_out, size_t &n_sc) {
+  const Sizes z = sizes_of(r.key);
+  if (r.kind == 0) {
+    n_in = z.flm;
+    n_out = z.wav;
+    n_sc = z.sc;
+  } else {
+    n_out = z.flm;
+    n_in = r.wav_from >= 0 ? 0 : z.wav;
+    n_sc = r.wav_from >= 0 ? 0 : z.sc;
+  }
+}
+
+// Bytes of those sizes the pool cannot supply.
+long long new_bytes(s2dw_hip_server *s, size_t a, size_t b, size_t c) {
+  std::vector<std::pair<size_t, void *>> took;
+  long long miss = 0;
+  for (size_t n : {a, b, c}) {
+    if (!n) continue;
+    void *q = s->dpool.take(n);
+    if (q) took.emplace_back(n, q);
+    else miss += (long long)n;
+  }
+  for (auto &t : took) s->dpool.put(t.second, t.first);
+  return miss;
+}
+
+// Takes a pooled buffer of n bytes or allocates one (n = 0: none).
+int dev_get(s2dw_hip_server *s, void **q, size_t n) {
+  *q = nullptr;
+  if (!n) return kErrNone;
+  *q = s->dpool.take(n);
+  if (*q) return kErrNone;
+  if (hipMalloc(q, n) != hipSuccess) {
+    *q = nullptr;
+    set_error("hipMalloc of %zu bytes failed", n);
+    return kErrMemAlloc;
+  }
+  s->in_use += (long long)n;
+  return kErrNone;
 }
 
 // Tries to admit the queue head; returns true if it was launched (or failed).
-bool admit_one(s2dw_hip_server *s) {
-  if (s->queue.empty() || (int)s->inflight.size() >= s->max_inflight) return false;
-  auto r = s->queue.front();
-  // a synthesis reading an analysis result: the producer was submitted
-  // earlier, so it is resident (finished) or still running; if neither, it
-  // failed and so does this request
-  const void *src_wav = nullptr, *src_sc = nullptr;
-  hipEvent_t src_ready = nullptr;
-  if (r->kind == 1 && r->wav_from >= 0) {
-    auto rt = s->resident.find(r->wav_from);
-    if (rt != s->resident.end()) {
-      src_wav = rt->second.d_wav;
-      src_sc = rt->second.d_sc;
-      src_ready = rt->second.ready;
-    } else {
-      for (auto &q : s->inflight)
-        if (q->ticket == r->wav_from && q->status == kErrNone) {
-          src_wav = q->d_out;
-          src_sc = q->d_sc;
-          src_ready = q->done;
-        }
+// Where the wavelet coefficients a synthesis reads come from.
+enum SrcState { kSrcReady, kSrcQueued, kSrcGone };
+
+SrcState source_of(s2dw_hip_server *s, const Request &r, const void **wav, const void **sc, hipEvent_t *ready) {
+  *wav = *sc = nullptr;
+  *ready = nullptr;
+  if (r.kind != 1 || r.wav_from < 0) return kSrcReady;
+  auto rt = s->resident.find(r.wav_from);
+  if (rt != s->resident.end()) {
+    *wav = rt->second.d_wav;
+    *sc = rt->second.d_sc;
+    *ready = rt->second.ready;
+    return kSrcReady;
+  }
+  for (auto &q : s->inflight)
+    if (q->ticket == r.wav_from) {
+      if (q->status != kErrNone) return kSrcGone;
+      *wav = q->d_out;
+      *sc = q->d_sc;
+      *ready = q->done;
+      return kSrcReady;
     }
-    if (!src_wav) {
-      s->queue.pop_front();
-      r->error = "the analysis this synthesis reads failed or was released";
-      r->status = kErrArgInvalid;
-      r->finished = true;
+  for (auto &q : s->queue)
+    if (q->ticket == r.wav_from) return kSrcQueued;
+  return kSrcGone;  // failed or released
+}
+
+bool admit_at(s2dw_hip_server *s, size_t qi, const void *src_wav, const void *src_sc, hipEvent_t src_ready);
+
+// Admits the first request, within a bounded look-ahead, that can start now:
+// its source (if it reads a resident analysis) is running or finished, and
+// an idle plan of its shape exists or none of its shape does (a request
+// waits for a busy plan of its shape rather than building a second one).
+// Returns true if a request was launched or failed.
+bool admit_one(s2dw_hip_server *s) {
+  if (s->queue.empty() || (int)s->inflight.size() >= (int)s->lanes.size()) return false;
+  constexpr size_t kLookahead = 64;
+  const size_t look = std::min(s->queue.size(), kLookahead);
+  for (size_t qi = 0; qi < look; ++qi) {
+    const auto &r = s->queue[qi];
+    const void *wav, *sc;
+    hipEvent_t ready;
+    const SrcState st = source_of(s, *r, &wav, &sc, &ready);
+    if (st == kSrcQueued) continue;
+    if (st == kSrcGone) {
+      auto rr = r;
+      s->queue.erase(s->queue.begin() + (long)qi);
+      rr->error = "the analysis this synthesis reads failed or was released";
+      rr->status = kErrArgInvalid;
+      rr->finished = true;
       s->cv_done.notify_all();
       return true;
     }
+    if (s->idle.find(r->key) == s->idle.end()) {
+      auto np = s->nplans.find(r->key);
+      if (np != s->nplans.end() && np->second > 0) continue;  // its plan is busy
+    }
+    return admit_at(s, qi, wav, sc, ready);
   }
-  const long long need_buf = request_bytes(*r);
+  return false;
+}
+
+bool admit_at(s2dw_hip_server *s, size_t qi, const void *src_wav, const void *src_sc, hipEvent_t src_ready) {
+  auto r = s->queue[qi];
+  size_t n_in = 0, n_out = 0, n_sc = 0;
+  dev_sizes(*r, n_in, n_out, n_sc);
+  auto fail = [&](int code, const std::string &msg) {
+    s->queue.erase(s->queue.begin() + (long)qi);
+    r->error = msg;
+    r->status = code;
+    r->finished = true;
+    s->cv_done.notify_all();
+    return true;
+  };
   PlanEntry *plan = nullptr;
   auto it = s->idle.find(r->key);
   if (it != s->idle.end()) {
     plan = it->second;
     s->idle.erase(it);
   }
-  auto evict_for = [&](long long need) {
-    while (s->in_use + need > s->budget && !s->idle.empty()) {
-      auto lru = s->idle.begin();
-      for (auto jt = s->idle.begin(); jt != s->idle.end(); ++jt)
-        if (jt->second->last_used < lru->second->last_used) lru = jt;
-      PlanEntry *victim = lru->second;
-      s->idle.erase(lru);
-      free_plan(s, victim);
-    }
-    return s->in_use + need <= s->budget;
-  };
   if (!plan) {
-    // a plan's size is known only once built: build, then check the budget
-    if (!evict_for(need_buf)) {
-      if (s->inflight.empty()) {  // nothing will free memory: the request cannot fit
-        s->queue.pop_front();
-        r->error = "request buffers exceed the server's HBM budget";
-        r->status = kErrMemAlloc;
-        r->finished = true;
-        s->cv_done.notify_all();
-        return true;
-      }
+    // a plan's size is known only once built: make room for the buffers,
+    // build, then check the budget again
+    if (!evict_for(s, new_bytes(s, n_in, n_out, n_sc))) {
+      if (s->inflight.empty()) return fail(kErrMemAlloc, "request buffers exceed the server's HBM budget");
       return false;
     }
     const int err = build_plan(r->key, &plan);
-    if (err) {
-      s->queue.pop_front();
-      r->error = s2dw_hip_last_error();
-      r->status = err;
-      r->finished = true;
-      s->cv_done.notify_all();
-      return true;
-    }
+    if (err) return fail(err, s2dw_hip_last_error());
     s->in_use += plan->p.device_bytes;
+    ++s->nplans[r->key];
   }
-  if (!evict_for(need_buf)) {
+  if (!evict_for(s, new_bytes(s, n_in, n_out, n_sc))) {
     if (!s->inflight.empty()) {
       // over budget for now: park the plan idle and retry when work retires
       plan->last_used = Clock::now();
@@ -234,46 +412,37 @@ bool admit_one(s2dw_hip_server *s) {
     }
     // nothing in flight and still over budget: the request can never fit
     free_plan(s, plan);
-    s->queue.pop_front();
     set_error("request B=%d N=%d needs more than the server's HBM budget of %lld bytes", r->key.B, r->key.N,
               s->budget);
-    r->error = s2dw_hip_last_error();
-    r->status = kErrMemAlloc;
-    r->finished = true;
-    s->cv_done.notify_all();
-    return true;
+    return fail(kErrMemAlloc, s2dw_hip_last_error());
   }
-  s->queue.pop_front();
+  s->queue.erase(s->queue.begin() + (long)qi);
   r->plan = plan;
   r->t_start = Clock::now();
   Plan &p = plan->p;
-  const long long wav_b = p.wav_doubles * 8, sc_b = 16LL * p.bl_scoeff * p.bl_scoeff, flm_b = 16LL * p.B * p.B;
-  int err = kErrNone;
-  auto alloc = [&](void **ptr, long long n) {
-    if (!err && hipMalloc(ptr, (size_t)std::max(1LL, n)) != hipSuccess) err = kErrMemAlloc;
-  };
-  if (r->kind == 0) {
-    alloc(&r->d_in, flm_b);
-    alloc(&r->d_out, wav_b);
-    alloc(&r->d_sc, sc_b);
-  } else {
-    alloc(&r->d_out, flm_b);
-    if (r->wav_from < 0) {
-      alloc(&r->d_in, wav_b);
-      alloc(&r->d_sc, sc_b);
+  for (int i = 0; i < (int)s->lanes.size(); ++i)
+    if (!s->lane_busy[i]) {
+      r->lane = i;
+      break;
     }
-  }
-  r->bytes = need_buf;
-  s->in_use += need_buf;
+  s->lane_busy[r->lane] = 1;  // a lane is free: inflight < lanes
+  p.stream = s->lanes[r->lane];
+  int err = dev_get(s, &r->d_in, n_in);
+  if (!err) err = dev_get(s, &r->d_out, n_out);
+  if (!err) err = dev_get(s, &r->d_sc, n_sc);
+  r->n_in = n_in;
+  r->n_out = n_out;
+  r->n_sc = n_sc;
   if (!err) err = hchk(hipEventCreateWithFlags(&r->done, hipEventDisableTiming), "event");
-  hipStream_t st = p.stream;
+  const hipStream_t st = p.stream;
+  // every copy is between device and pinned memory: asynchronous
   if (!err && r->kind == 0) {
-    err = hchk(hipMemcpyAsync(r->d_in, r->in_host, flm_b, hipMemcpyHostToDevice, st), "copy in");
+    err = hchk(hipMemcpyAsync(r->d_in, r->h_in.ptr, n_in, hipMemcpyHostToDevice, st), "copy in");
     if (!err) err = run_analysis(&p, (double *)r->d_out, (double2 *)r->d_sc, (const double2 *)r->d_in);
-    if (!err && r->out_host)
-      err = hchk(hipMemcpyAsync(r->out_host, r->d_out, wav_b, hipMemcpyDeviceToHost, st), "copy out");
-    if (!err && r->out_host2)
-      err = hchk(hipMemcpyAsync(r->out_host2, r->d_sc, sc_b, hipMemcpyDeviceToHost, st), "copy out");
+    if (!err && r->h_out.ptr)
+      err = hchk(hipMemcpyAsync(r->h_out.ptr, r->d_out, n_out, hipMemcpyDeviceToHost, st), "copy out");
+    if (!err && r->h_out2.ptr)
+      err = hchk(hipMemcpyAsync(r->h_out2.ptr, r->d_sc, n_sc, hipMemcpyDeviceToHost, st), "copy out");
   } else if (!err) {
     const void *wav = r->d_in, *sc = r->d_sc;
     if (r->wav_from >= 0) {
@@ -281,11 +450,11 @@ bool admit_one(s2dw_hip_server *s) {
       sc = src_sc;
       err = hchk(hipStreamWaitEvent(st, src_ready, 0), "wait producer");
     } else {
-      err = hchk(hipMemcpyAsync(r->d_in, r->in_host, wav_b, hipMemcpyHostToDevice, st), "copy in");
-      if (!err) err = hchk(hipMemcpyAsync(r->d_sc, r->in_host2, sc_b, hipMemcpyHostToDevice, st), "copy in");
+      err = hchk(hipMemcpyAsync(r->d_in, r->h_in.ptr, n_in, hipMemcpyHostToDevice, st), "copy in");
+      if (!err) err = hchk(hipMemcpyAsync(r->d_sc, r->h_in2.ptr, n_sc, hipMemcpyHostToDevice, st), "copy in");
     }
     if (!err) err = run_synthesis(&p, (double2 *)r->d_out, (const double *)wav, (const double2 *)sc);
-    if (!err) err = hchk(hipMemcpyAsync(r->out_host, r->d_out, flm_b, hipMemcpyDeviceToHost, st), "copy out");
+    if (!err) err = hchk(hipMemcpyAsync(r->h_out.ptr, r->d_out, n_out, hipMemcpyDeviceToHost, st), "copy out");
   }
   if (!err) err = hchk(hipEventRecord(r->done, st), "event");
   r->status = err;
@@ -296,26 +465,29 @@ bool admit_one(s2dw_hip_server *s) {
 
 void finish(s2dw_hip_server *s, const std::shared_ptr<Request> &r) {
   r->t_end = Clock::now();
+  if (r->status != kErrNone && r->lane >= 0) {
+    // the lane may still run what was enqueued before the failure
+    (void)hipStreamSynchronize(s->lanes[r->lane]);
+  }
+  if (r->lane >= 0) s->lane_busy[r->lane] = 0;
+  r->lane = -1;
   if (r->kind == 0 && r->keep_resident && r->status == kErrNone) {
     Resident res;
     res.d_wav = r->d_out;
     res.d_sc = r->d_sc;
-    res.bytes = r->bytes;
+    res.n_wav = r->n_out;
+    res.n_sc = r->n_sc;
     res.ready = r->done;  // already complete; later consumers wait on it trivially
     res.key = r->key;
     s->resident[r->ticket] = res;
-    r->d_out = r->d_sc = nullptr;
+    dev_put(s, r->d_in, r->n_in);
+    r->d_in = r->d_out = r->d_sc = nullptr;
     r->done = nullptr;
-    (void)hipFree(r->d_in);
-    r->d_in = nullptr;
-    const long long flm_b = 16LL * r->key.B * r->key.B;
-    s->resident[r->ticket].bytes = r->bytes - flm_b;
-    s->in_use -= flm_b;
   } else {
-    (void)hipFree(r->d_in);
-    (void)hipFree(r->d_out);
-    (void)hipFree(r->d_sc);
-    s->in_use -= r->bytes;
+    dev_put(s, r->d_in, r->n_in);
+    dev_put(s, r->d_out, r->n_out);
+    dev_put(s, r->d_sc, r->n_sc);
+    r->d_in = r->d_out = r->d_sc = nullptr;
     if (r->done) (void)hipEventDestroy(r->done);
     r->done = nullptr;
   }
@@ -359,8 +531,10 @@ int submit(s2dw_hip_server *s, std::shared_ptr<Request> r, long long *ticket) {
     set_error("invalid sizes B=%d N=%d J=%d alpha=%g", r->key.B, r->key.N, r->key.J, r->key.alpha);
     return kErrSizeInvalid;
   }
-  std::lock_guard<std::mutex> g(s->mu);
+  const Clock::time_point t0 = Clock::now();
   if (r->kind == 1 && r->wav_from >= 0) {
+    // checked again at admission (the source may be released meanwhile)
+    std::lock_guard<std::mutex> g(s->mu);
     auto it = s->resident.find(r->wav_from);
     auto pend = s->all.find(r->wav_from);
     const bool ok = (it != s->resident.end() && it->second.key.B == r->key.B && it->second.key.N == r->key.N &&
@@ -372,8 +546,31 @@ int submit(s2dw_hip_server *s, std::shared_ptr<Request> r, long long *ticket) {
       return kErrArgInvalid;
     }
   }
+  // stage the host data in pinned buffers here, on the client's thread
+  const Sizes z = sizes_of(r->key);
+  bool ok = true;
+  if (r->kind == 0) {
+    ok = pin_get(s, r->h_in, z.flm) && pin_get(s, r->h_out, r->out_host ? z.wav : 0) &&
+         pin_get(s, r->h_out2, r->out_host2 ? z.sc : 0);
+    if (ok) std::memcpy(r->h_in.ptr, r->in_host, z.flm);
+  } else {
+    ok = pin_get(s, r->h_out, z.flm);
+    if (ok && r->wav_from < 0) {
+      ok = pin_get(s, r->h_in, z.wav) && pin_get(s, r->h_in2, z.sc);
+      if (ok) {
+        std::memcpy(r->h_in.ptr, r->in_host, z.wav);
+        std::memcpy(r->h_in2.ptr, r->in_host2, z.sc);
+      }
+    }
+  }
+  if (!ok) {
+    pin_release_all(s, *r);
+    set_error("pinned host staging of %zu bytes failed", z.wav + z.flm + z.sc);
+    return kErrMemAlloc;
+  }
+  std::lock_guard<std::mutex> g(s->mu);
   r->ticket = s->next_ticket++;
-  r->t_submit = Clock::now();
+  r->t_submit = t0;  // latency includes the staging copy
   s->all[r->ticket] = r;
   s->queue.push_back(r);
   *ticket = r->ticket;
@@ -398,6 +595,17 @@ int s2dw_hip_server_create(s2dw_hip_server_t *srv, long long hbm_budget_bytes, i
   }
   s->budget = hbm_budget_bytes;
   s->max_inflight = max_inflight;
+  const int nl = std::min(max_inflight, kMaxLanes);
+  for (int i = 0; i < nl; ++i) {
+    hipStream_t q = nullptr;
+    if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) {
+      for (hipStream_t o : s->lanes) (void)hipStreamDestroy(o);
+      delete s;
+      return kErrInitFail;
+    }
+    s->lanes.push_back(q);
+    s->lane_busy.push_back(0);
+  }
   s->worker = std::thread(worker_loop, s);
   *srv = s;
   return kErrNone;
@@ -417,6 +625,10 @@ int s2dw_hip_server_destroy(s2dw_hip_server_t s) {
     if (kv.second.ready) (void)hipEventDestroy(kv.second.ready);
   }
   for (auto &kv : s->idle) free_plan(s, kv.second);
+  for (auto &kv : s->all) pin_release_all(s, *kv.second);  // never waited for
+  for (auto &kv : s->dpool.free) (void)hipFree(kv.second);
+  for (hipStream_t q : s->lanes) (void)hipStreamDestroy(q);
+  for (auto &kv : s->hpool.free) (void)hipHostFree(kv.second);
   delete s;
   return kErrNone;
 }
@@ -449,14 +661,23 @@ int s2dw_hip_server_submit_synthesis(s2dw_hip_server_t s, int B, int N, int J, d
 
 int s2dw_hip_server_wait(s2dw_hip_server_t s, long long ticket, double *latency_ms, double *run_ms) {
   if (!s) return kErrArgInvalid;
-  std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->all.find(ticket);
-  if (it == s->all.end()) return kErrArgInvalid;
-  auto r = it->second;
-  s->cv_done.wait(lk, [&] { return r->finished; });
+  std::shared_ptr<Request> r;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return kErrArgInvalid;
+    r = it->second;
+    s->cv_done.wait(lk, [&] { return r->finished; });
+    s->all.erase(ticket);
+  }
+  // outputs from the pinned staging buffers, on the caller's thread
+  if (r->status == kErrNone) {
+    if (r->h_out.ptr && r->out_host) std::memcpy(r->out_host, r->h_out.ptr, r->h_out.n);
+    if (r->h_out2.ptr && r->out_host2) std::memcpy(r->out_host2, r->h_out2.ptr, r->h_out2.n);
+  }
+  pin_release_all(s, *r);
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_end - r->t_submit).count();
   if (run_ms) *run_ms = std::chrono::duration<double, std::milli>(r->t_end - r->t_start).count();
-  s->all.erase(ticket);
   if (r->status != kErrNone) set_error("%s", r->error.c_str());
   return r->status;
 }
@@ -471,10 +692,9 @@ int s2dw_hip_server_release(s2dw_hip_server_t s, long long ticket) {
     if (q->wav_from == ticket) return kErrArgInvalid;
   for (auto &q : s->inflight)
     if (q->wav_from == ticket) return kErrArgInvalid;
-  (void)hipFree(it->second.d_wav);
-  (void)hipFree(it->second.d_sc);
+  dev_put(s, it->second.d_wav, it->second.n_wav);
+  dev_put(s, it->second.d_sc, it->second.n_sc);
   if (it->second.ready) (void)hipEventDestroy(it->second.ready);
-  s->in_use -= it->second.bytes;
   s->resident.erase(it);
   return kErrNone;
 }
