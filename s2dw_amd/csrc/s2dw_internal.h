@@ -145,6 +145,7 @@ struct Plan {
   std::vector<int> smf_lb, smf_ub;
   std::vector<long long> smf_btoff;
   double *d_BT = nullptr;     // per-scale synthesis B tables
+  long long bt_doubles = 0;
   double2 *d_cc = nullptr;    // m'-recursion coefficients [m'][l]
   double *d_cp = nullptr;     // checkpoint planes [chunk][2][m][l]
   double2 *d_fpart = nullptr; // chunk partial sums [chunk][m][l]
