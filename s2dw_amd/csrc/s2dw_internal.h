@@ -111,6 +111,7 @@ struct Plan {
   int *d_lmeta = nullptr;     // per l: flush info (int2), then per group of 4 l: end bits
   int *d_gev = nullptr;       // inside d_lmeta
   double *d_lcoef = nullptr;  // l-recursion coefficients [m'][l] (A, Bc)
+  double *d_uinv = nullptr;   // [l][m] 1/sqrt((l+1)^2 - m^2) (0 where not positive)
   int4 *d_sblk = nullptr;     // synthesis l-blocks (l_first, count, slot-0 scale, slot-1 scale)
   int n_sblk = 0;
   double2 *d_G = nullptr;     // analysis gamma matrix [g][c]
