@@ -315,6 +315,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second")
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of replaying "
                                                          "captured HIP graphs")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel paths")
     args = ap.parse_args()
     if args.traffic_from:
         traffic_from(*args.traffic_from)
@@ -348,6 +349,7 @@ def main():
     d_f = torch.from_numpy(np.ascontiguousarray(flm.T)).to(dev)
     d_w, d_sc, d_g = plan.empty_wav(dev), plan.empty_scoeff(dev), plan.empty_flm(dev)
 
+    tuned = plan.autotune(3) if not args.no_autotune else "not run"
     # graph replay (captured at the first warmup step) unless --eager
     plan.set_graphs(not args.eager)
     for _ in range(max(1, args.warmup)):
@@ -438,7 +440,8 @@ def main():
         "data": "synthetic: seeded real-signal flm, Re/Im ~ N(0,1) scaled by (l+1)^-1",
         "config": {"workload": "analysis+synthesis round trip, L=512 N=8 alpha=2 J_min=2 (J=7)", "L": L, "N": N,
                    "alpha": ALPHA, "J": J, "bl_scoeff": bl, "precision": args.precision,
-                   "transforms_per_step": 2, "launch": "eager" if args.eager else "hip graph replay"},
+                   "transforms_per_step": 2, "launch": "eager" if args.eager else "hip graph replay",
+                   "autotune": tuned},
         "roofline": roof,
         "latency_ms": {"analysis_p50": float(np.percentile(lat_a, 50)),
                        "analysis_p99": float(np.percentile(lat_a, 99)),

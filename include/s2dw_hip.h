@@ -180,6 +180,18 @@ int s2dw_hip_plan_set_timing(s2dw_hip_plan_t plan, int enable);
  * eagerly.  No counterpart in the reference (which has no launch cost to
  * amortise). */
 int s2dw_hip_plan_set_graphs(s2dw_hip_plan_t plan, int enable);
+
+/* Autotuner (no counterpart in the reference, which has one code path).
+ * Times, on the device, the alternatives a built plan can switch between:
+ * analysis contraction (FP64 MFMA with independent waves, MFMA with
+ * workgroup-staged operands, VALU with 2 or 1 chains per lane), synthesis
+ * contraction (MFMA, VALU with 4 or 8 chains per lane) and the helper stream
+ * for the small scales (on/off), each candidate over `reps` transforms on
+ * temporary zero-filled buffers, and keeps the fastest.  Results stay within
+ * the FP64 parity tolerance whichever path is chosen.  If report is not
+ * NULL, up to report_len - 1 characters of the measured times and the
+ * choices are written there.  Blocks until done; drops captured graphs. */
+int s2dw_hip_plan_autotune(s2dw_hip_plan_t plan, int reps, char *report, int report_len);
 /* Sums (ms) and launch counts per stage of everything recorded since the
  * last call; waits for the recorded work.  ms and count hold 9 entries. */
 int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count);

@@ -114,6 +114,7 @@ def lib() -> C.CDLL:
         L.s2dw_hip_wigner_plane.argtypes = [C.c_int, _dp, _dp]
         L.s2dw_hip_plan_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.s2dw_hip_plan_set_graphs.argtypes = [C.c_void_p, C.c_int]
+        L.s2dw_hip_plan_autotune.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
         L.s2dw_hip_plan_stage_times.argtypes = [C.c_void_p, _dp, _ip]
         _lib = L
     return _lib
@@ -326,6 +327,13 @@ class Plan:
 
     def set_timing(self, enable: bool):
         _check(lib().s2dw_hip_plan_set_timing(self.h, int(bool(enable))), "s2dw_hip_plan_set_timing")
+
+    def autotune(self, reps: int = 3) -> str:
+        """Time the switchable kernel paths on the device and keep the fastest;
+        returns the measured times and the choices."""
+        buf = C.create_string_buffer(1024)
+        _check(lib().s2dw_hip_plan_autotune(self.h, int(reps), buf, len(buf)), "s2dw_hip_plan_autotune")
+        return buf.value.decode()
 
     def set_graphs(self, enable: bool):
         """Replay each (direction, buffers) combination as a captured HIP graph."""

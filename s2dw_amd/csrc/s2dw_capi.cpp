@@ -329,6 +329,17 @@ int s2dw_hip_plan_set_timing(s2dw_hip_plan_t plan, int enable) {
   return kErrNone;
 }
 
+int s2dw_hip_plan_autotune(s2dw_hip_plan_t plan, int reps, char *report, int report_len) {
+  if (!plan) return kErrArgInvalid;
+  std::string r;
+  const int e = plan_autotune(&plan->p, reps, &r);
+  if (report && report_len > 0) {
+    std::strncpy(report, r.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = '\0';
+  }
+  return e;
+}
+
 int s2dw_hip_plan_set_graphs(s2dw_hip_plan_t plan, int enable) {
   if (!plan) return kErrArgInvalid;
   plan->p.graphs = enable != 0;

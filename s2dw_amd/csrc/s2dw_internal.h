@@ -154,6 +154,7 @@ struct Plan {
   int ana_p = 2;              // chains per lane in the VALU analysis contraction
   bool ana_mfma = true;       // analysis contraction on FP64 MFMA
   bool ana_wave_indep = true; // ... with independent waves (else workgroup-staged operands)
+  bool ana_mfma_ok = true, syn_mfma_ok = true;  // the MFMA contractions are available on this plan
   int fft_engine = 0;         // 0 workgroup engine, 1 wave engine, 2 generic Stockham
   // HIP graphs of whole transforms for the device API: one executable graph
   // per (direction, buffers), captured on `cap` at first use and replayed
@@ -179,6 +180,7 @@ int run_synthesis(Plan *p, double2 *d_flm, const double *d_wav, const double2 *d
 // run_analysis (kind 0) or run_synthesis (kind 1) through the plan's graph
 // cache when graphs are on and stage timing is off, eagerly otherwise
 int run_graphed(Plan *p, int kind, void *out0, void *out1, const void *in);
+int plan_autotune(Plan *p, int reps, std::string *report);
 void plan_drop_graphs(Plan *p);
 int stage_times(Plan *p, double *ms, int *count);
 int pack_static(Plan *p, const double *d_packed, double *d_static, bool to_static);

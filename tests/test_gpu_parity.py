@@ -234,6 +234,34 @@ def test_graph_replay_matches_eager(B, N, stream):
     plan.close()
 
 
+@pytest.mark.parametrize("B,N", [(64, 5), (48, 3)])
+def test_autotuned_plan_matches_default(B, N):
+    """After the autotuner has picked its kernel paths, the plan's wavelet
+    coefficients agree with the default paths (host API) within the FP64
+    tolerance, and the round trip reconstructs f_lm."""
+    import torch
+
+    alpha = 2.0
+    say(f"autotune B={B} N={N}")
+    st = setup(B, N, alpha)
+    f = seeded_flm(B, 77)
+    plan = core.Plan(B, N, st.J, alpha, st.K, st.Phi2, st.S)
+    report = plan.autotune(2)
+    say("  " + report)
+    assert "analysis contraction:" in report and "chosen:" in report
+    d_f = torch.from_numpy(np.ascontiguousarray(f.T)).cuda()
+    d_w, d_sc, d_g = plan.empty_wav(), plan.empty_scoeff(), plan.empty_flm()
+    plan.analysis(d_f, d_w, d_sc)
+    plan.synthesis(d_w, d_sc, d_g)
+    plan.sync()
+    w_host, _ = core.s2dw_core_analysis_flm2wav_dynamic(f, st.K, st.S, st.J, B, N, st.bl_scoeff, alpha)
+    for j in range(st.J + 1):
+        wj = plan.scale_view(d_w, j).cpu().numpy()
+        assert np.abs(np.transpose(wj, (2, 1, 0)) - w_host[j]).max() <= TOL
+    assert np.abs(d_g.cpu().numpy().T - f).max() <= TOL
+    plan.close()
+
+
 def test_invalid_bl_scoeff_raises():
     st = setup(32, 3, 2.0)
     f = seeded_flm(32, 1)
