@@ -298,6 +298,54 @@ def load_traffic(stage):
     return sum(h["hbm_bytes"] * h["dispatches"] for h in hits) / sum(h["dispatches"] for h in hits)
 
 
+def bench_pipelined(args, L, N, J, K, Phi2, S, prec, d_f, flm):
+    """Round trips with the analysis of step i+1 overlapping the synthesis of
+    step i: one plan per direction, each on its own stream, wavelet and
+    scaling coefficients double-buffered, events for the two dependencies
+    (synthesis i reads what analysis i wrote; analysis i+2 overwrites what
+    synthesis i read).  Returns transforms/s over the same number of steps."""
+    import torch
+
+    from s2dw_amd import core
+
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    pa = core.Plan(L, N, J, ALPHA, K, Phi2, S, precision=prec, stream=sa)
+    ps = core.Plan(L, N, J, ALPHA, K, Phi2, S, precision=prec, stream=sb)
+    for p in (pa, ps):
+        if not args.no_autotune:
+            p.autotune(2)
+        p.set_graphs(not args.eager)
+    d_w = [pa.empty_wav(), pa.empty_wav()]
+    d_sc = [pa.empty_scoeff(), pa.empty_scoeff()]
+    d_g = ps.empty_flm()
+    e_ana = [torch.cuda.Event(), torch.cuda.Event()]
+    e_syn = [torch.cuda.Event(), torch.cuda.Event()]
+
+    def run(n):
+        for i in range(n):
+            k = i & 1
+            sa.wait_event(e_syn[k]) if i >= 2 else None
+            pa.analysis(d_f, d_w[k], d_sc[k])
+            e_ana[k].record(sa)
+            sb.wait_event(e_ana[k])
+            ps.synthesis(d_w[k], d_sc[k], d_g)
+            e_syn[k].record(sb)
+
+    torch.cuda.synchronize()
+    run(max(2, args.warmup))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    err = float(np.abs(d_g.cpu().numpy().T - flm).max())
+    pa.close()
+    ps.close()
+    return {"transforms_per_s": 2 * args.steps / wall, "ms_per_step": 1e3 * wall / args.steps,
+            "round_trip_max_err": err,
+            "note": "analysis of step i+1 overlapping synthesis of step i (two plans, two streams)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -316,6 +364,8 @@ def main():
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of replaying "
                                                          "captured HIP graphs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel paths")
+    ap.add_argument("--pipeline", action="store_true", help="also measure round trips with the analysis of step "
+                                                            "i+1 overlapping the synthesis of step i")
     args = ap.parse_args()
     if args.traffic_from:
         traffic_from(*args.traffic_from)
@@ -371,6 +421,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
+    pipelined = bench_pipelined(args, L, N, J, K, Phi2, S, prec, d_f, flm) if args.pipeline else None
     # second pass: HIP events around every stage launch, on the plan's stream
     plan.set_timing(True)
     for i in range(steps):
@@ -449,6 +500,7 @@ def main():
                        "synthesis_p99": float(np.percentile(lat_s, 99)),
                        "p50": float(np.percentile(lat_all, 50)), "p99": float(np.percentile(lat_all, 99))},
         "round_trip_max_err": rt_err,
+        "pipelined": pipelined,
         "plan_device_bytes": plan.device_bytes,
         "stages": stages,
         "contraction": contraction,
