@@ -185,7 +185,8 @@ int s2dw_hip_plan_set_graphs(s2dw_hip_plan_t plan, int enable);
  * Times, on the device, the alternatives a built plan can switch between:
  * analysis contraction (FP64 MFMA with independent waves, MFMA with
  * workgroup-staged operands, VALU with 2 or 1 chains per lane), synthesis
- * contraction (MFMA, VALU with 4 or 8 chains per lane) and the helper stream
+ * contraction (MFMA with independent waves or a staged V tile, VALU with 4 or
+ * 8 chains per lane) and the helper stream
  * for the small scales (on/off), each candidate over `reps` transforms on
  * temporary zero-filled buffers, and keeps the fastest.  Results stay within
  * the FP64 parity tolerance whichever path is chosen.  If report is not

@@ -141,6 +141,7 @@ struct Plan {
   std::vector<hipEvent_t> ev_pool;   // free events
   int syn_q = 4;              // chains per lane in the VALU synthesis contraction
   bool syn_mfma = true;       // synthesis contraction on FP64 MFMA
+  bool syn_wave = true;       // ... with independent waves (else a V tile staged in LDS)
   int smf_NCp = 0, smf_CW = 0, smf_nch = 0, n_swork = 0;
   int smf_wcount[3] = {0, 0, 0};  // work items per k-step class (<=2, <=4, <=8)
   std::vector<int> smf_lb, smf_ub;
