@@ -67,14 +67,23 @@ def contraction_cmacs(hi, lo, N):
     return ana, syn
 
 
-def stage_bytes(hi, lo, N, ncs):
-    """Compulsory HBM bytes per stage launch group (all scales) for one transform."""
-    b = {}
-    sc = [(h, l_) for h, l_ in zip(hi, lo) if l_ + 1 <= h - 1]
-    b["ana_beta_fft"] = sum(N * h * h * 16 + N * 2 * h * h * 16 for h, _ in sc)
-    b["ana_alpha_c2r"] = sum(N * 2 * h * h * 16 + N * 2 * h * (2 * h - 1) * 8 for h, _ in sc)
-    b["syn_alpha_r2c"] = sum(N * 2 * h * (2 * h - 1) * 8 + N * 2 * h * h * 16 for h, _ in sc)
-    b["syn_beta_fft"] = sum(N * 2 * h * h * 16 + ncs * h * h * 16 for h, _ in sc)
+def stage_bytes(hi, lo, N, ncs, grouped=True):
+    """Compulsory HBM bytes per stage for one transform: reading each stage's
+    input once and writing its output once.  With grouped launches the FFT
+    stages 'ana_beta_fft' ... 'syn_beta_fft' cover scales 0 and 1 and the
+    'small' stages cover scales 2..J; otherwise the FFT stages cover all."""
+    def beta_a(h): return N * h * h * 16 + N * 2 * h * h * 16
+    def alpha_a(h): return N * 2 * h * h * 16 + N * 2 * h * (2 * h - 1) * 8
+    def alpha_s(h): return N * 2 * h * (2 * h - 1) * 8 + N * 2 * h * h * 16
+    def beta_s(h): return N * 2 * h * h * 16 + ncs * h * h * 16
+    sc = [j for j, (h, l_) in enumerate(zip(hi, lo)) if l_ + 1 <= h - 1]
+    big = [hi[j] for j in sc if j < 2 or not grouped]
+    small = [hi[j] for j in sc if j >= 2 and grouped]
+    b = {"ana_beta_fft": sum(map(beta_a, big)), "ana_alpha_c2r": sum(map(alpha_a, big)),
+         "syn_alpha_r2c": sum(map(alpha_s, big)), "syn_beta_fft": sum(map(beta_s, big))}
+    if small:
+        b["ana_small_ffts"] = sum(beta_a(h) + alpha_a(h) for h in small)
+        b["syn_small_ffts"] = sum(alpha_s(h) + beta_s(h) for h in small)
     return b
 
 
@@ -281,8 +290,9 @@ def bench_serve(args):
 
 # stage name -> kernel whose launches dominate it
 STAGE_KERNEL = {"ana_contract": "k_ana_contract_mfma_w", "syn_contract": "k_syn_contract_mfma",
-                "ana_beta_fft": "k_beta_ana_t", "ana_alpha_c2r": "k_alpha_c2r_t",
-                "syn_alpha_r2c": "k_alpha_r2c_t", "syn_beta_fft": "k_beta_syn_t"}
+                "ana_beta_fft": "k_beta_ana_t2", "ana_alpha_c2r": "k_alpha_c2r_t2",
+                "syn_alpha_r2c": "k_alpha_r2c_t2", "syn_beta_fft": "k_beta_syn_t2",
+                "ana_small_ffts": "k_alpha_c2r_g", "syn_small_ffts": "k_alpha_r2c_g"}
 
 
 def load_traffic(stage):
@@ -439,7 +449,8 @@ def main():
 
     hi, lo = plan.bl_hi, plan.bl_lo
     ana_cmac, syn_cmac = contraction_cmacs(hi, lo, N)
-    sb = stage_bytes(hi, lo, N, ncs=sum(1 for n in range(N) if (N + n) % 2))
+    grouped = st.get("ana_small_ffts", (0.0, 0))[1] > 0
+    sb = stage_bytes(hi, lo, N, ncs=sum(1 for n in range(N) if (N + n) % 2), grouped=grouped)
     stages = {}
     for name, (ms, cnt) in st.items():
         if cnt:

@@ -166,10 +166,12 @@ void *s2dw_hip_plan_stream(s2dw_hip_plan_t plan);
  * 0 analysis f_lm relayout, 1 analysis l-contraction (incl. gamma-DFT),
  * 2 analysis beta FFTs, 3 analysis alpha c2r FFTs, 4 scaling coefficients,
  * 5 synthesis alpha r2c FFTs, 6 synthesis gamma + beta quadrature FFTs,
- * 7 synthesis Wigner contraction, 8 synthesis f_lm assembly.
+ * 7 synthesis Wigner contraction, 8 synthesis f_lm assembly, 9 analysis FFTs
+ * of the grouped small scales (j >= 2), 10 synthesis FFTs of the grouped
+ * small scales (stages 2, 3, 5, 6 then cover the other scales).
  * No counterpart in the reference (which prints wall-clock only,
  * s2dw_test.f90:97-135). */
-#define S2DW_NUM_STAGES 9
+#define S2DW_NUM_STAGES 11
 int s2dw_hip_plan_set_timing(s2dw_hip_plan_t plan, int enable);
 /* Graph replay for s2dw_hip_analysis_device / s2dw_hip_synthesis_device
  * (enable != 0): the first call with a given set of buffers captures the
@@ -194,7 +196,7 @@ int s2dw_hip_plan_set_graphs(s2dw_hip_plan_t plan, int enable);
  * choices are written there.  Blocks until done; drops captured graphs. */
 int s2dw_hip_plan_autotune(s2dw_hip_plan_t plan, int reps, char *report, int report_len);
 /* Sums (ms) and launch counts per stage of everything recorded since the
- * last call; waits for the recorded work.  ms and count hold 9 entries. */
+ * last call; waits for the recorded work.  ms and count hold 11 entries. */
 int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count);
 
 /* ---------------------------------------------------------------------

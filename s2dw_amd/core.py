@@ -323,7 +323,7 @@ class Plan:
         _check(lib().s2dw_hip_plan_sync(self.h), "s2dw_hip_plan_sync")
 
     STAGES = ["ana_relayout", "ana_contract", "ana_beta_fft", "ana_alpha_c2r", "ana_scoeff", "syn_alpha_r2c",
-              "syn_beta_fft", "syn_contract", "syn_assemble"]
+              "syn_beta_fft", "syn_contract", "syn_assemble", "ana_small_ffts", "syn_small_ffts"]
 
     def set_timing(self, enable: bool):
         _check(lib().s2dw_hip_plan_set_timing(self.h, int(bool(enable))), "s2dw_hip_plan_set_timing")

@@ -83,7 +83,7 @@ struct Column {
 };
 
 // Optional per-stage HIP-event timing (stage ids in include/s2dw_hip.h).
-constexpr int kNumStages = 9;
+constexpr int kNumStages = 11;
 struct StageEvents {
   int stage;
   hipEvent_t a, b;
