@@ -548,7 +548,9 @@ __device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const T
 // registers; its radix-R0 stage comes last (Ns = M/R0) and multiplies its
 // outputs by post_last[k] (k < M) if given.  Saves one LDS write, one LDS
 // read and a barrier pair per transform against two separate FFTs.
-template <int E, int NT, int DIR>
+// FIRST_DONE: the caller has already written the outputs of the forward
+// transform's first stage (radix 2^(log2 M mod 3), Ns = 1) to the slots.
+template <int E, int NT, int DIR, bool FIRST_DONE = false>
 __device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, int M, int logTw,
                                                const double2 *__restrict__ post_last) {
   constexpr int NB = E / 8;
@@ -565,8 +567,10 @@ __device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, i
     for (int r = 0; r < 8; ++r) pm[q][r] = F[jj + r * nR];
   }
   // forward
-  if (r0 == 1) team_stage<2, E, NT, -1>(buf, M, logTw, 1, s.tw, nullptr);
-  else if (r0 == 2) team_stage<4, E, NT, -1>(buf, M, logTw, 1, s.tw, nullptr);
+  if (!FIRST_DONE) {
+    if (r0 == 1) team_stage<2, E, NT, -1>(buf, M, logTw, 1, s.tw, nullptr);
+    else if (r0 == 2) team_stage<4, E, NT, -1>(buf, M, logTw, 1, s.tw, nullptr);
+  }
   int Ns = 1 << r0;
 #pragma unroll
   for (int st = 0; st < 2; ++st) {
@@ -644,11 +648,11 @@ __device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, i
 // chirp_k^{DIR} otherwise).  M, logTw and bluestein are passed separately
 // from s so that kernels specialised on the work length hand down
 // compile-time constants (every function here is inlined).
-template <int E, int NT, int DIR>
+template <int E, int NT, int DIR, bool FIRST_DONE = false>
 __device__ __forceinline__ void team_fft_core(const FftSpec &s, double2 *buf, int M, int logTw, bool bluestein,
                                               const double2 *__restrict__ post_last = nullptr) {
   if (bluestein && ilog2_dev(M) >= 3) {
-    team_bluestein<E, NT, DIR>(s, buf, M, logTw, post_last);
+    team_bluestein<E, NT, DIR, FIRST_DONE>(s, buf, M, logTw, post_last);
     return;
   }
   TeamTw<E> T;
