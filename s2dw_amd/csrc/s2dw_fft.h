@@ -550,9 +550,13 @@ __device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const T
 // read and a barrier pair per transform against two separate FFTs.
 // FIRST_DONE: the caller has already written the outputs of the forward
 // transform's first stage (radix 2^(log2 M mod 3), Ns = 1) to the slots.
-template <int E, int NT, int DIR, bool FIRST_DONE = false>
-__device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, int M, int logTw,
-                                               const double2 *__restrict__ post_last) {
+// LAST_REGS (radix-4 last stage only): the backward transform's last stage
+// leaves outputs jj + r M/4, r < 2, of its butterflies in yl (butterfly
+// b = threadIdx.x + NT q: sequence b mod Tw, jj = b / Tw) instead of the
+// slots, without post_last; the slots are not written by that stage.
+template <int E, int NT, int DIR, bool FIRST_DONE, bool LAST_REGS>
+__device__ __forceinline__ void team_bluestein_x(const FftSpec &s, double2 *buf, int M, int logTw,
+                                                 const double2 *__restrict__ post_last, double2 (&yl)[E / 4][2]) {
   constexpr int NB = E / 8;
   const int logM = ilog2_dev(M), nst = logM / 3, r0 = logM % 3;
   const double2 *F = DIR > 0 ? s.filt_b : s.filt_f;
@@ -628,6 +632,23 @@ __device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, i
       Ns *= 8;
     }
   }
+  if constexpr (LAST_REGS) {
+    // radix-4 stage at Ns = M/4: butterfly jj reads jj + r M/4 and writes
+    // the same positions; twiddle w^r with w = tw[jj]
+#pragma unroll
+    for (int q = 0; q < E / 4; ++q) {
+      const int b = threadIdx.x + NT * q;
+      const int t = b & (Tw - 1), jj = b >> logTw, nq = M >> 2;
+      double2 x[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) x[r] = buf[lpad(((jj + r * nq) << logTw) + t)];
+      twiddle_powers<4, +1>(x, s.tw[jj], s.tw[2 * jj], x[0], x[0]);
+      dft_pow2<4, +1>(x);
+      yl[q][0] = x[0];
+      yl[q][1] = x[1];
+    }
+    return;
+  }
   if (r0 == 1) team_stage<2, E, NT, +1>(buf, M, logTw, Ns, s.tw, post_last);
   else if (r0 == 2) team_stage<4, E, NT, +1>(buf, M, logTw, Ns, s.tw, post_last);
   else if (post_last) {
@@ -638,6 +659,13 @@ __device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, i
     }
     lds_barrier();
   }
+}
+
+template <int E, int NT, int DIR, bool FIRST_DONE = false>
+__device__ __forceinline__ void team_bluestein(const FftSpec &s, double2 *buf, int M, int logTw,
+                                               const double2 *__restrict__ post_last) {
+  double2 yl[E / 4][2];
+  team_bluestein_x<E, NT, DIR, FIRST_DONE, false>(s, buf, M, logTw, post_last, yl);
 }
 
 // Middle of a length-n transform on the workgroup engine.  Direct power of
