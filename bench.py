@@ -447,6 +447,25 @@ def main():
     g = d_g.cpu().numpy().T
     rt_err = float(np.abs(g - flm).max())
 
+    # per-scale moments of the analysis output (s2dw_stat_moments) on the
+    # device: one read of the packed coefficients, HIP events on the plan's stream
+    plan.analysis(d_f, d_w, d_sc)
+    d_mo = torch.empty((8, J + 1), dtype=torch.float64, device=dev)
+    for _ in range(2):
+        plan.stat_moments(d_w, d_out=d_mo)
+    m0, m1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    m0.record(stream)
+    for _ in range(steps):
+        plan.stat_moments(d_w, d_out=d_mo)
+    m1.record(stream)
+    torch.cuda.synchronize()
+    mo_ms = m0.elapsed_time(m1) / steps
+    mo_bytes = 8.0 * plan.wav_doubles
+    moments = {"ms": mo_ms, "GBps": mo_bytes / (mo_ms * 1e-3) / 1e9,
+               "frac_hbm_peak": mo_bytes / (mo_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes": mo_bytes,
+               "kernels": ["k_mom_partial", "k_mom_final"], "mean": d_mo[0].cpu().tolist(),
+               "var": d_mo[1].cpu().tolist()}
+
     hi, lo = plan.bl_hi, plan.bl_lo
     ana_cmac, syn_cmac = contraction_cmacs(hi, lo, N)
     grouped = st.get("ana_small_ffts", (0.0, 0))[1] > 0
@@ -516,6 +535,7 @@ def main():
         "stages": stages,
         "contraction": contraction,
         "fft_stages": ffts,
+        "moments": moments,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(K, Phi2, S, J, bl, flm)

@@ -200,6 +200,36 @@ int s2dw_hip_plan_autotune(s2dw_hip_plan_t plan, int reps, char *report, int rep
 int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count);
 
 /* ---------------------------------------------------------------------
+ * Moments of the wavelet coefficients of each scale: s2dw_stat_moments,
+ * s2dw_stat_mod.f90:81-275 (called after an analysis by s2dw_wstats.f90:
+ * 181-190).  Per scale jj: mean, variance (n-1 normalised), skewness, excess
+ * kurtosis, fifth and excess sixth standardised moments of the
+ * (2hi-1)*2hi*N coefficients.  With a mask (same shapes): the coefficients
+ * are multiplied by it in place, nj = nint(sum of the mask), and the sums run
+ * over the entries whose mask is 1 within TOL_ZERO (stat_mod.f90:136-150).
+ * A scale whose variance is below TOL_ZERO gets zero skewness and higher
+ * moments (stat_mod.f90:251-260).
+ * ------------------------------------------------------------------- */
+
+/* Host pointers, synchronous; arguments in the order of s2dw_stat_moments.
+ * wavdyn[jj]: (2hi-1)*2hi*N doubles (masked in place when mask != NULL);
+ * mask: NULL or mask[jj] of the same shapes; mean..kur: J+1 doubles each;
+ * five, six: J+1 doubles or NULL (the optional outputs).  Returns
+ * S2DW_ERROR_SIZE_INVALID if a scale has nj < 2 (the reference halts),
+ * S2DW_ERROR_ARTH_WARNING (results complete) if a variance is below
+ * TOL_ZERO. */
+int s2dw_hip_stat_moments(double *const *wavdyn, int J, int B, int N, double alpha, double *mean, double *var,
+                          double *skew, double *kur, const double *const *mask, double *five, double *six);
+
+/* Device pointers, enqueued on the plan's stream, no allocation: d_wav is
+ * the plan's packed coefficients (as s2dw_hip_analysis_device writes them;
+ * masked in place when d_mask != NULL), d_mask NULL or packed alike; both
+ * 16-byte aligned.  d_out (device, 8*(J+1) doubles) receives rows
+ * d_out[k*(J+1) + jj]: k = 0 mean, 1 var, 2 skew, 3 kur, 4 five, 5 six,
+ * 6 nj, 7 flag (0 ok, 1 variance below TOL_ZERO, 2 nj < 2: values NaN). */
+int s2dw_hip_stat_moments_device(s2dw_hip_plan_t plan, double *d_wav, const double *d_mask, double *d_out);
+
+/* ---------------------------------------------------------------------
  * Request loop (no counterpart in the reference, whose routines run one call
  * at a time from one host thread, s2dw_core_mod.f90).  A server owns a worker
  * thread, a pool of plans and an HBM budget.  Requests of any (B, N, J,

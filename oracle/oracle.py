@@ -60,6 +60,8 @@ def lib():
         L.orc_synthesis_static.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int,
                                            C.c_int, C.c_double]
         L.orc_test_gen_flm.argtypes = [C.c_int, _dp, C.c_int]
+        L.orc_stat_moments.argtypes = [C.POINTER(_dp), C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
+                                       _dp, C.POINTER(_dp), _dp, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_dl_precise.argtypes = [C.c_int]
         _lib = L
@@ -209,6 +211,18 @@ def synthesis_static(wav, scoeff, K, Phi2, S, J, B, N, bl_scoeff, alpha):
     if err:
         raise RuntimeError(f"orc_synthesis_static error {err}")
     return flm
+
+
+def stat_moments(wavs, J, B, N, alpha, masks=None):
+    """s2dw_stat_moments (s2dw_stat_mod.f90:81-275) on per-scale arrays
+    ``wavs[jj]`` ((2hi-1) x 2hi x N, order 'F', float64), masked in place when
+    ``masks`` is given.  Returns (code, mean, var, skew, kur, five, six)."""
+    n = J + 1
+    wp = (_dp * n)(*[_p(w) for w in wavs])
+    mp = (_dp * n)(*[_p(m) for m in masks]) if masks is not None else None
+    out = [np.zeros(n) for _ in range(6)]
+    code = lib().orc_stat_moments(wp, J, B, N, alpha, *[_p(o) for o in out[:4]], mp, _p(out[4]), _p(out[5]))
+    return (code, *out)
 
 
 class Setup:

@@ -1,10 +1,11 @@
 !------------------------------------------------------------------------------
 ! ref_dump -- driver (project code) around the reference's own Fortran modules
-! s2dw_types_mod, s2dw_error_mod, s2dw_dl_mod and s2dw_core_mod, compiled from
-! the reference sources by oracle/Makefile into oracle/_ref/.  TEST
-! INFRASTRUCTURE ONLY: it writes golden vectors of the reference's setup
-! routines and Wigner planes for tests/golden/.  It never calls the reference's
-! analysis/synthesis routines (they need FFTW, which is absent here).
+! s2dw_types_mod, s2dw_error_mod, s2dw_dl_mod, s2dw_core_mod and s2dw_stat_mod,
+! compiled from the reference sources by oracle/Makefile into oracle/_ref/.
+! TEST INFRASTRUCTURE ONLY: it writes golden vectors of the reference's setup
+! routines, Wigner planes and wavelet-coefficient moments for tests/golden/.
+! It never calls the reference's analysis/synthesis routines (they need FFTW,
+! which is absent here).
 !
 ! usage (all output is little-endian stream binary to <out>):
 !   ref_dump kernels B alpha J out    -> J_max, bl_scoeff, pass, K_gamma, Phi2, admiss
@@ -12,12 +13,18 @@
 !   ref_dump dl      l out            -> dl(-l:l,-l:l) at beta = pi/2
 !   ref_dump dlbeta  l beta out       -> dl(-l:l,-l:l) at beta
 !   ref_dump bands   B alpha J out    -> J_max, then (hi, lo) per scale
+!   ref_dump moments B N alpha J masked in out
+!       in: the coefficients of scales 0..J ((2hi-1)*2hi*N each, in order),
+!       then, if masked = 1, the masks alike; out: mean, var, skew, kur, five,
+!       six (J+1 each) from s2dw_stat_moments, then the coefficients after the
+!       call (masked in place when masked = 1)
 !------------------------------------------------------------------------------
 program ref_dump
 
   use s2dw_types_mod
   use s2dw_core_mod
   use s2dw_dl_mod
+  use s2dw_stat_mod
 
   implicit none
 
@@ -27,7 +34,9 @@ program ref_dump
   real(dp), allocatable :: K_gamma(:,:), Phi2(:), admiss(:), dl(:,:)
   complex(dpc), allocatable :: Slm(:,:)
   logical :: pass
-  integer :: ipass
+  integer :: ipass, masked
+  type(s2dw_wav_abg), allocatable :: wavdyn(:), wavmask(:)
+  real(dp), allocatable :: mean(:), var(:), skew(:), kur(:), five(:), six(:)
 
   call get_command_argument(1, mode)
 
@@ -96,6 +105,45 @@ program ref_dump
            bl_lo = max(floor(B / (alpha**(jj+1))), 0)
         end if
         write(u) bl_hi, bl_lo
+     end do
+     close(u)
+
+  case ('moments')
+     call get_command_argument(2, arg); read(arg,*) B
+     call get_command_argument(3, arg); read(arg,*) N
+     call get_command_argument(4, arg); read(arg,*) alpha
+     call get_command_argument(5, arg); read(arg,*) J
+     call get_command_argument(6, arg); read(arg,*) masked
+     call get_command_argument(7, arg)
+     call get_command_argument(8, outf)
+     allocate(wavdyn(0:J), wavmask(0:J))
+     allocate(mean(0:J), var(0:J), skew(0:J), kur(0:J), five(0:J), six(0:J))
+     open(newunit=u, file=trim(arg), access='stream', form='unformatted', status='old')
+     do jj = 0, J
+        if (s2dw_core_assimilate_int(B / (alpha**(jj-1)), bl_hi)) then
+           bl_hi = min(bl_hi, B)
+        else
+           bl_hi = min(ceiling(B / (alpha**(jj-1))), B)
+        end if
+        allocate(wavdyn(jj)%coeff(0:2*bl_hi-2, 0:2*bl_hi-1, 0:N-1))
+        allocate(wavmask(jj)%coeff(0:2*bl_hi-2, 0:2*bl_hi-1, 0:N-1))
+        read(u) wavdyn(jj)%coeff
+     end do
+     if (masked == 1) then
+        do jj = 0, J
+           read(u) wavmask(jj)%coeff
+        end do
+     end if
+     close(u)
+     if (masked == 1) then
+        call s2dw_stat_moments(wavdyn, J, B, N, alpha, mean, var, skew, kur, wavmask, five, six)
+     else
+        call s2dw_stat_moments(wavdyn, J, B, N, alpha, mean, var, skew, kur, five=five, six=six)
+     end if
+     open(newunit=u, file=trim(outf), access='stream', form='unformatted', status='replace')
+     write(u) mean, var, skew, kur, five, six
+     do jj = 0, J
+        write(u) wavdyn(jj)%coeff
      end do
      close(u)
 

@@ -44,6 +44,7 @@ typedef double complex cplx;
 /* s2dw_types_mod.f90:48-60 (TOL_ADMISS and TOL_ZERO are default-real
  * literals, so they carry single-precision rounding). */
 #define ORC_TOL_ADMISS ((double)1e-12f)
+#define ORC_TOL_ZERO ((double)1e-12f)
 #define ORC_TOL_QUAD 1e-12
 #define ORC_TOL_LIMIT 1e-12
 #define ORC_TOL_CEIL 1e-5
@@ -56,6 +57,7 @@ enum {
   ORC_ERR_NONE = 0,
   ORC_ERR_MEM_ALLOC_FAIL = 4,
   ORC_ERR_ARTH = 5,
+  ORC_ERR_ARTH_WARNING = 6,
   ORC_ERR_SIZE_INVALID = 8,
   ORC_ERR_QUAD_STEP_EXCEED = 12,
   ORC_ERR_ADMISS_FAIL = 13
@@ -931,4 +933,87 @@ void orc_test_gen_flm(int B, cplx *flm, int seed) {
       flm[el + (size_t)B * m] = re + I * im;
     }
   }
+}
+
+/* s2dw_stat_moments, s2dw_stat_mod.f90:81-275, one scale after another (the
+ * reference splits the scales over OpenMP threads).  wavdyn[jj] holds
+ * (2hi-1)*2hi*N values (band limits as 108-119) and is multiplied by mask[jj]
+ * in place when mask is not NULL (136-140).  Every sum runs in array order
+ * over the entries whose mask is 1 within TOL_ZERO (153-197), or over all
+ * entries without a mask (201-242).  five and six may be NULL (the optional
+ * outputs).  Returns ORC_ERR_SIZE_INVALID (a halting error in the reference)
+ * when nj < 2 (143-150), ORC_ERR_ARTH_WARNING when some variance is below
+ * TOL_ZERO (results complete, 251-260). */
+int orc_stat_moments(double *const *wavdyn, int J, int B, int N, double alpha, double *mean, double *var,
+                     double *skew, double *kur, double *const *mask, double *five, double *six) {
+  int warn = 0;
+  for (int jj = 0; jj <= J; jj++) {
+    int hi, lo;
+    orc_band_limits(jj, B, alpha, &hi, &lo);
+    const size_t cnt = (size_t)(2 * hi - 1) * (size_t)(2 * hi) * (size_t)N;
+    double *w = wavdyn[jj];
+    const double *mk = mask ? mask[jj] : NULL;
+    if (mk)
+      for (size_t i = 0; i < cnt; i++) w[i] = w[i] * mk[i];
+    long nj;
+    if (mk) {
+      double sm = 0;
+      for (size_t i = 0; i < cnt; i++) sm += mk[i];
+      nj = lround(sm); /* nint: halves away from zero */
+    } else {
+      nj = (long)cnt;
+    }
+    if (nj < 2) return ORC_ERR_SIZE_INVALID;
+#define ORC_IN(i) (!mk || fabs(mk[i] - 1.0) < ORC_TOL_ZERO)
+    double sum = 0;
+    for (size_t i = 0; i < cnt; i++)
+      if (ORC_IN(i)) sum += w[i];
+    const double mu = sum / nj;
+    double ep = 0, v = 0, sk = 0, ku = 0, f5 = 0, s6 = 0;
+    for (size_t i = 0; i < cnt; i++)
+      if (ORC_IN(i)) ep += w[i] - mu;
+    for (size_t i = 0; i < cnt; i++)
+      if (ORC_IN(i)) {
+        const double s1 = w[i] - mu;
+        v += s1 * s1;
+      }
+    for (size_t i = 0; i < cnt; i++)
+      if (ORC_IN(i)) {
+        const double s1 = w[i] - mu;
+        sk += s1 * s1 * s1;
+      }
+    for (size_t i = 0; i < cnt; i++)
+      if (ORC_IN(i)) {
+        const double s1 = w[i] - mu;
+        ku += s1 * s1 * s1 * s1;
+      }
+    for (size_t i = 0; i < cnt; i++)
+      if (ORC_IN(i)) {
+        const double s1 = w[i] - mu;
+        f5 += s1 * s1 * s1 * s1 * s1;
+      }
+    for (size_t i = 0; i < cnt; i++)
+      if (ORC_IN(i)) {
+        const double s1 = w[i] - mu;
+        s6 += s1 * s1 * s1 * s1 * s1 * s1;
+      }
+#undef ORC_IN
+    v = (v - ep * ep / (double)nj) / (double)(nj - 1);
+    const double sd = sqrt(v);
+    mean[jj] = mu;
+    var[jj] = v;
+    if (v < ORC_TOL_ZERO) {
+      warn = 1;
+      skew[jj] = 0;
+      kur[jj] = 0;
+      if (five) five[jj] = 0;
+      if (six) six[jj] = 0;
+    } else {
+      skew[jj] = sk / (nj * (sd * sd * sd));
+      kur[jj] = ku / (nj * (v * v)) - 3.0;
+      if (five) five[jj] = f5 / (nj * (sd * sd * sd * sd * sd));
+      if (six) six[jj] = s6 / (nj * (v * v * v)) - 15.0;
+    }
+  }
+  return warn ? ORC_ERR_ARTH_WARNING : ORC_ERR_NONE;
 }

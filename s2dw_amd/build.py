@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libs2dw_hip.so")
-SOURCES = ["s2dw_setup.cpp", "s2dw_kernels.hip", "s2dw_capi.cpp", "s2dw_serve.cpp"]
+SOURCES = ["s2dw_setup.cpp", "s2dw_kernels.hip", "s2dw_capi.cpp", "s2dw_serve.cpp", "s2dw_stats.hip"]
 ARCH = os.environ.get("S2DW_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -42,7 +42,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     os.makedirs(OBJ_DIR, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fopenmp", "-o", tmp, *objs]

@@ -116,6 +116,9 @@ def lib() -> C.CDLL:
         L.s2dw_hip_plan_set_graphs.argtypes = [C.c_void_p, C.c_int]
         L.s2dw_hip_plan_autotune.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
         L.s2dw_hip_plan_stage_times.argtypes = [C.c_void_p, _dp, _ip]
+        L.s2dw_hip_stat_moments.argtypes = [C.POINTER(_dp), C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
+                                            _dp, C.POINTER(_dp), _dp, _dp]
+        L.s2dw_hip_stat_moments_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -237,6 +240,52 @@ def s2dw_core_synthesis_wav2flm_dynamic(wavdyn, scoeff, K_gamma, Phi2, Slm, J, B
     return flm
 
 
+def s2dw_stat_moments(wavdyn, J, B, N, alpha, wavdyn_mask=None):
+    """s2dw_stat_moments, s2dw_stat_mod.f90:81-275 -> (mean, var, skew, kur, five, six),
+    each of length J+1.
+
+    With ``wavdyn_mask`` (per-scale arrays shaped like ``wavdyn``) the
+    coefficients are multiplied by the mask in place, as the Fortran routine
+    does with its intent(inout) argument.  nj < 2 on a scale raises
+    :class:`S2dwError` (a halting error in the reference); a variance below
+    TOL_ZERO emits a warning (the reference's non-halting
+    S2DW_ERROR_ARTH_WARNING) and returns zero skewness and higher moments for
+    that scale."""
+    n = int(J) + 1
+    if len(wavdyn) != n or (wavdyn_mask is not None and len(wavdyn_mask) != n):
+        raise S2dwError(S2DW_ERROR_ARG_INVALID, "s2dw_stat_moments", "one array per scale 0..J is required")
+    hi, _ = band_limits(J, B, alpha)
+    wavs, masks = [], []
+    for j in range(n):
+        shape = (2 * hi[j] - 1, 2 * hi[j], N)
+        w = np.asfortranarray(wavdyn[j], dtype=np.float64)
+        if w.shape != shape:
+            raise S2dwError(S2DW_ERROR_SIZE_INVALID, "s2dw_stat_moments", f"scale {j}: shape {w.shape} != {shape}")
+        wavs.append(w)
+        if wavdyn_mask is not None:
+            m = np.asfortranarray(wavdyn_mask[j], dtype=np.float64)
+            if m.shape != shape:
+                raise S2dwError(S2DW_ERROR_SIZE_INVALID, "s2dw_stat_moments", f"mask {j}: shape {m.shape}")
+            masks.append(m)
+    wp = (_dp * n)(*[_p(w) for w in wavs])
+    mp = (_dp * n)(*[_p(m) for m in masks]) if masks else None
+    out = [np.zeros(n) for _ in range(6)]
+    code = lib().s2dw_hip_stat_moments(wp, n - 1, int(B), int(N), float(alpha), *[_p(o) for o in out[:4]], mp,
+                                       _p(out[4]), _p(out[5]))
+    if masks:
+        for w, w0 in zip(wavs, wavdyn):
+            if w is not w0:
+                np.copyto(w0, w)
+    if code == S2DW_ERROR_ARTH_WARNING:
+        import warnings
+
+        warnings.warn(str(S2dwError(code, "s2dw_stat_moments", lib().s2dw_hip_last_error().decode())),
+                      RuntimeWarning, stacklevel=2)
+    else:
+        _check(code, "s2dw_stat_moments")
+    return tuple(out)
+
+
 def wigner_plane(l: int, which: str = "both"):
     """Delta^l_{m'm}(pi/2), 0 <= m', m <= l, from the device recursions:
     (l-recursion plane, m'-recursion plane), each indexed [m', m]."""
@@ -321,6 +370,22 @@ class Plan:
 
     def sync(self):
         _check(lib().s2dw_hip_plan_sync(self.h), "s2dw_hip_plan_sync")
+
+    MOMENT_ROWS = ["mean", "var", "skew", "kur", "five", "six", "nj", "flag"]
+
+    def stat_moments(self, d_wav, d_mask=None, d_out=None):
+        """Enqueue the per-scale moments of a packed wav (s2dw_stat_moments) on
+        the plan's stream; returns d_out, a (8, J+1) float64 device tensor with
+        rows MOMENT_ROWS (flag 1: variance below TOL_ZERO, 2: nj < 2).  With
+        d_mask the coefficients of d_wav are masked in place."""
+        import torch
+
+        if d_out is None:
+            d_out = torch.empty((8, self.J + 1), dtype=torch.float64, device=d_wav.device)
+        mask = C.c_void_p(d_mask.data_ptr()) if d_mask is not None else None
+        _check(lib().s2dw_hip_stat_moments_device(self.h, C.c_void_p(d_wav.data_ptr()), mask,
+                                                  C.c_void_p(d_out.data_ptr())), "s2dw_hip_stat_moments_device")
+        return d_out
 
     STAGES = ["ana_relayout", "ana_contract", "ana_beta_fft", "ana_alpha_c2r", "ana_scoeff", "syn_alpha_r2c",
               "syn_beta_fft", "syn_contract", "syn_assemble", "ana_small_ffts", "syn_small_ffts"]

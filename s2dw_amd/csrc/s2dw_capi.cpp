@@ -7,11 +7,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <list>
 #include <memory>
 #include <mutex>
+#include <vector>
 
 #include "../../include/s2dw_hip.h"
 #include "s2dw_internal.h"
@@ -350,6 +352,105 @@ int s2dw_hip_plan_set_graphs(s2dw_hip_plan_t plan, int enable) {
 int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count) {
   if (!plan || !ms || !count) return kErrArgInvalid;
   return stage_times(&plan->p, ms, count);
+}
+
+// ---------------------------------------------------------------------------
+// Moments of the wavelet coefficients (s2dw_stat_moments,
+// s2dw_stat_mod.f90:81-275).
+// ---------------------------------------------------------------------------
+
+int s2dw_hip_stat_moments_device(s2dw_hip_plan_t plan, double *d_wav, const double *d_mask, double *d_out) {
+  if (!plan || !d_wav || !d_out) return kErrArgInvalid;
+  if (((uintptr_t)d_wav & 15) || ((uintptr_t)d_mask & 15)) {
+    set_error("s2dw_hip_stat_moments_device: d_wav and d_mask must be 16-byte aligned");
+    return kErrArgInvalid;
+  }
+  Plan &p = plan->p;
+  const int n = p.J + 1;
+  std::vector<long long> off(n), cnt(n);
+  for (int j = 0; j < n; ++j) {
+    off[j] = p.sc[j].wav_off;
+    cnt[j] = (long long)(2 * p.sc[j].hi - 1) * 2 * p.sc[j].hi * p.N;
+  }
+  return moments_run(p.stream, off.data(), cnt.data(), n, d_wav, d_mask, p.d_mom, d_out);
+}
+
+namespace {
+struct DevBufs {
+  std::vector<void *> v;
+  hipStream_t st = nullptr;
+  ~DevBufs() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (void *q : v) (void)hipFree(q);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  int get(double **out, size_t doubles) {
+    void *q = nullptr;
+    const hipError_t e = hipMalloc(&q, sizeof(double) * (doubles > 0 ? doubles : 1));
+    if (e != hipSuccess) return hip_error(e, "hipMalloc");
+    v.push_back(q);
+    *out = static_cast<double *>(q);
+    return kErrNone;
+  }
+};
+}  // namespace
+
+int s2dw_hip_stat_moments(double *const *wavdyn, int J, int B, int N, double alpha, double *mean, double *var,
+                          double *skew, double *kur, const double *const *mask, double *five, double *six) {
+  if (!wavdyn || !mean || !var || !skew || !kur || J < 0 || B < 1 || N < 1 || !(alpha > 1.0))
+    return kErrArgInvalid;
+  const int n = J + 1;
+  for (int j = 0; j < n; ++j)
+    if (!wavdyn[j] || (mask && !mask[j])) return kErrArgInvalid;
+  std::vector<long long> off(n), cnt(n);
+  long long total = 0;
+  for (int j = 0; j < n; ++j) {
+    int hi = 0, lo = 0;
+    band_limits(j, B, alpha, &hi, &lo);
+    cnt[j] = (long long)(2 * hi - 1) * 2 * hi * N;
+    off[j] = total;
+    total += cnt[j];
+  }
+  DevBufs d;
+  S2DW_HIP_CHECK(hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking));
+  double *d_w = nullptr, *d_m = nullptr, *d_part = nullptr, *d_out = nullptr;
+  int err;
+  if ((err = d.get(&d_w, (size_t)total))) return err;
+  if (mask && (err = d.get(&d_m, (size_t)total))) return err;
+  if ((err = d.get(&d_part, (size_t)moments_partial_doubles(cnt.data(), n)))) return err;
+  if ((err = d.get(&d_out, 8 * (size_t)n))) return err;
+  for (int j = 0; j < n; ++j) {
+    S2DW_HIP_CHECK(hipMemcpyAsync(d_w + off[j], wavdyn[j], sizeof(double) * cnt[j], hipMemcpyHostToDevice, d.st));
+    if (mask)
+      S2DW_HIP_CHECK(hipMemcpyAsync(d_m + off[j], mask[j], sizeof(double) * cnt[j], hipMemcpyHostToDevice, d.st));
+  }
+  if ((err = moments_run(d.st, off.data(), cnt.data(), n, d_w, d_m, d_part, d_out))) return err;
+  std::vector<double> out(8 * (size_t)n);
+  S2DW_HIP_CHECK(hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, d.st));
+  // with a mask the coefficients are returned masked (s2dw_stat_mod.f90:136-140)
+  if (mask)
+    for (int j = 0; j < n; ++j)
+      S2DW_HIP_CHECK(hipMemcpyAsync(wavdyn[j], d_w + off[j], sizeof(double) * cnt[j], hipMemcpyDeviceToHost, d.st));
+  S2DW_HIP_CHECK(hipStreamSynchronize(d.st));
+  int warn = -1;
+  for (int j = 0; j < n; ++j) {
+    if (out[7 * (size_t)n + j] == 2) {
+      set_error("s2dw_stat_moments: scale %d has %.0f samples; at least two are required", j, out[6 * (size_t)n + j]);
+      return kErrSizeInvalid;
+    }
+    if (out[7 * (size_t)n + j] == 1 && warn < 0) warn = j;
+    mean[j] = out[j];
+    var[j] = out[(size_t)n + j];
+    skew[j] = out[2 * (size_t)n + j];
+    kur[j] = out[3 * (size_t)n + j];
+    if (five) five[j] = out[4 * (size_t)n + j];
+    if (six) six[j] = out[5 * (size_t)n + j];
+  }
+  if (warn >= 0) {
+    set_error("s2dw_stat_moments: skewness/kurtosis & higher undefined when variance zero (scale %d)", warn);
+    return kErrArthWarning;
+  }
+  return kErrNone;
 }
 
 }  // extern "C"

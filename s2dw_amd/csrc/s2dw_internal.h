@@ -19,6 +19,7 @@ enum {
   kErrInitFail = 3,
   kErrMemAlloc = 4,
   kErrArth = 5,
+  kErrArthWarning = 6,
   kErrSizeInvalid = 8,
   kErrArgInvalid = 10,
   kErrQuadStepExceed = 12,
@@ -169,6 +170,8 @@ struct Plan {
   void *groups = nullptr;  // grouped launches of the small scales (kernels' own state)
   bool graphs = false;
   hipStream_t cap = nullptr;
+  double *d_mom = nullptr;    // moments partials (allocated at first use)
+  long long mom_doubles = 0;
   std::vector<GraphEntry> gcache;
   unsigned long long gtick = 0;
 };
@@ -185,6 +188,13 @@ int plan_autotune(Plan *p, int reps, std::string *report);
 void plan_drop_graphs(Plan *p);
 int stage_times(Plan *p, double *ms, int *count);
 int pack_static(Plan *p, const double *d_packed, double *d_static, bool to_static);
+// Moments of packed per-scale coefficients (s2dw_stats.hip): scale s holds
+// count[s] (even) doubles at d_wav + off[s]; d_mask (same packing) or NULL;
+// d_part holds moments_partial_doubles(count, nscales) doubles; d_out gets
+// 8 rows of nscales values (mean, var, skew, kur, five, six, nj, flag).
+long long moments_partial_doubles(const long long *count, int nscales);
+int moments_run(hipStream_t st, const long long *off, const long long *count, int nscales, double *d_wav,
+                const double *d_mask, double *d_part, double *d_out);
 int wigner_plane(int l, double *out, double *out2);
 
 }  // namespace s2dw

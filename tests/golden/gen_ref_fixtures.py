@@ -1,16 +1,18 @@
 """Write golden vectors of the reference's own setup routines and Wigner planes.
 
-Runs oracle/_ref/ref_dump (the reference's s2dw_core_mod / s2dw_dl_mod compiled
-from its Fortran sources by ``make -C oracle ref``) and stores its outputs as
+Runs oracle/_ref/ref_dump (the reference's s2dw_core_mod / s2dw_dl_mod /
+s2dw_stat_mod compiled from its Fortran sources by ``make -C oracle ref``) and
+stores its outputs as
 small .npz fixtures beside this script.  Run in a container that holds the
 reference sources; the fixtures are committed, the GPU box only reads them.
 
-    python tests/golden/gen_ref_fixtures.py
+    python tests/golden/gen_ref_fixtures.py [--moments-only]
 """
 from __future__ import annotations
 
 import os
 import subprocess
+import sys
 import tempfile
 
 import numpy as np
@@ -42,9 +44,69 @@ def jmax_of(B, alpha):
     return int(np.frombuffer(b[:4], dtype="<i4")[0])
 
 
+# s2dw_stat_moments cases: (name, B, N, alpha, data, mask); J = J_max.
+MOMENT_CASES = [("plain", 16, 3, 2.0, "normal", None), ("mask", 16, 3, 2.0, "normal", "binary"),
+                ("frac", 16, 3, 2.0, "normal", "fractional"), ("const", 8, 2, 2.0, "const1", None),
+                ("offset", 27, 1, 3.0, "offset_exp", None), ("gamma", 16, 2, 2.0, "gamma", "binary")]
+
+
+def scale_his(B, alpha, J):
+    bb = np.frombuffer(run("bands", B, alpha, J), dtype="<i4")
+    return [int(h) for h in bb[1:].reshape(J + 1, 2)[:, 0]]
+
+
+def moments_fixtures():
+    out = {}
+    rng = np.random.default_rng(2012)
+    for name, B, N, alpha, data, mask in MOMENT_CASES:
+        J = jmax_of(B, alpha)
+        sizes = [(2 * h - 1) * 2 * h * N for h in scale_his(B, alpha, J)]
+        tot = sum(sizes)
+        if data == "normal":
+            x = rng.standard_normal(tot)
+        elif data == "offset_exp":
+            x = 1000.0 + rng.exponential(1.0, tot)
+        elif data == "gamma":
+            x = rng.gamma(2.0, 0.5, tot) - 1.0
+        else:  # "const1": scale 1 constant (zero variance), the others normal
+            x = rng.standard_normal(tot)
+            x[sizes[0]:sizes[0] + sizes[1]] = 0.25
+        m = None
+        if mask == "binary":
+            m = (rng.random(tot) < 0.75).astype(np.float64)
+        elif mask == "fractional":
+            m = rng.choice(np.array([0.0, 0.5, 1.0, 1.0 + 5e-13, 1.0]), size=tot)
+        with tempfile.TemporaryDirectory() as td:
+            inp = os.path.join(td, "in.bin")
+            with open(inp, "wb") as fh:
+                fh.write(x.astype("<f8").tobytes())
+                if m is not None:
+                    fh.write(m.astype("<f8").tobytes())
+            b = run("moments", B, N, alpha, J, 1 if m is not None else 0, inp)
+        d = np.frombuffer(b, dtype="<f8")
+        k = f"{name}_"
+        out[k + "meta"] = np.array([B, N, J], dtype=np.int64)
+        out[k + "alpha"] = np.array([alpha])
+        out[k + "x"] = x
+        if m is not None:
+            out[k + "mask"] = m
+        out[k + "moments"] = d[:6 * (J + 1)].reshape(6, J + 1)
+        after = d[6 * (J + 1):]
+        if m is None:
+            assert np.array_equal(after, x), name  # unmasked: the input is left alone
+        else:
+            out[k + "x_after"] = after
+    np.savez_compressed(os.path.join(HERE, "ref_moments.npz"), **out)
+
+
 def main():
     if not os.path.exists(REF_DUMP):
         raise SystemExit("build the reference driver first: make -C oracle ref")
+    if "--moments-only" in sys.argv:
+        moments_fixtures()
+        print("fixtures written to", HERE)
+        return
+    moments_fixtures()
     # --- kernels / admissibility / band limits
     kern = {}
     for B, alpha, J in KERNEL_CASES:

@@ -9,12 +9,14 @@ import re
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
 
 FDIR = os.path.join(ROOT, "s2dw_amd", "fortran")
 HIP_TEST = os.path.join(FDIR, "_build", "s2dw_hip_test")
+STATS_TEST = os.path.join(FDIR, "_build", "s2dw_hip_stats_test")
 REF_TEST = os.path.join(ROOT, "oracle", "_ref", "s2dw_test_hip")
 
 REFERENCE_PUBLIC = ["s2dw_core_init_kernels", "s2dw_core_init_directionality", "s2dw_core_comp_Jmax",
@@ -35,9 +37,18 @@ def test_module_public_list_matches_reference():
 def test_modules_build():
     r = subprocess.run(["make", "-C", FDIR], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    for m in ("s2dw_types_mod", "s2dw_error_mod", "s2dw_core_mod", "s2dw_fileio_mod"):
+    for m in ("s2dw_types_mod", "s2dw_error_mod", "s2dw_core_mod", "s2dw_fileio_mod", "s2dw_stat_mod"):
         assert os.path.exists(os.path.join(FDIR, "_build", m + ".mod"))
-    assert os.path.exists(HIP_TEST)
+    assert os.path.exists(HIP_TEST) and os.path.exists(STATS_TEST)
+
+
+def test_stat_module_interface_matches_reference():
+    # s2dw_stat_moments keeps the reference's dummy arguments
+    # (src/mod/s2dw_stat_mod.f90:81-94)
+    src = open(os.path.join(FDIR, "s2dw_stat_mod.f90")).read()
+    assert re.search(r"subroutine s2dw_stat_moments\(wavdyn, J, B, N, alpha, mean, var, skew, kur, wavdyn_mask, "
+                     r"five, six\)", src)
+    assert "type(s2dw_wav_abg), intent(in), optional, target :: wavdyn_mask(0:J)" in src
 
 
 def run(cmd):
@@ -75,3 +86,45 @@ def test_reference_harness_unchanged_on_gpu_library(B):
     assert "Tests passed!" in r.stdout
     err = float(re.search(r"Average max error\s*=\s*([0-9.Ee+-]+)", r.stdout).group(1))
     assert err < 1e-12
+
+
+def stats_inputs(B, N, alpha):
+    """The coefficients and mask s2dw_hip_stats_test builds (see its header)."""
+    import oracle as O
+
+    J = O.comp_Jmax(B, alpha)
+    his, _ = O.band_limits(J, B, alpha)
+    wavs, masks = [], []
+    for jj, h in enumerate(his):
+        k = np.arange((2 * h - 1) * 2 * h * N, dtype=np.float64)
+        x = np.sin(0.37 * k + jj) + 0.1 * np.cos(1.3 * k) ** 3
+        wavs.append(np.asfortranarray(x.reshape((2 * h - 1, 2 * h, N), order="F")))
+        m = np.where(np.arange(k.size) % 7 == 3, 0.0, 1.0)
+        masks.append(np.asfortranarray(m.reshape((2 * h - 1, 2 * h, N), order="F")))
+    return J, wavs, masks
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("B,N,alpha,mode", [(32, 3, 2.0, 0), (32, 3, 2.0, 1), (27, 2, 3.0, 1)])
+def test_fortran_stat_moments(B, N, alpha, mode):
+    import oracle as O
+    from test_moments_oracle import assert_moments
+
+    r = run([STATS_TEST, str(B), str(N), str(alpha), str(mode)])
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = np.array([[float(v) for v in line.split()] for line in r.stdout.strip().splitlines()])
+    J, wavs, masks = stats_inputs(B, N, alpha)
+    assert rows.shape == (J + 1, 8)
+    code, *want = O.stat_moments(wavs, J, B, N, alpha, masks if mode else None)
+    assert code == 0
+    assert_moments([rows[:, 1 + k] for k in range(6)], want, f"fortran mode {mode}")
+    sums = np.array([w.sum() for w in wavs])  # after the oracle applied the mask in place
+    np.testing.assert_allclose(rows[:, 7], sums, rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.gpu
+def test_fortran_stat_moments_halts_like_reference():
+    # an all-zero mask: nj < 2 (s2dw_stat_mod.f90:147-150)
+    r = run([STATS_TEST, "16", "2", "2.0", "2"])
+    assert "Require at least two samples to compute moments" in r.stdout
+    assert "Halting program execution" in r.stdout
