@@ -18,6 +18,11 @@
 !       then, if masked = 1, the masks alike; out: mean, var, skew, kur, five,
 !       six (J+1 each) from s2dw_stat_moments, then the coefficients after the
 !       call (masked in place when masked = 1)
+!   ref_dump momwrite nsim J nopt out
+!       the reference's s2dw_stat_moments_write into the text file out, for
+!       the values v(isim, jj, k) = (-1)**k * (isim + 1.5) * 10**(jj - 3) / (k + 3)
+!       (k = 0 mean .. 5 six), descriptions "set<isim>", and nopt = 0, 1, 2
+!       of the optional five, six passed
 !------------------------------------------------------------------------------
 program ref_dump
 
@@ -37,6 +42,9 @@ program ref_dump
   integer :: ipass, masked
   type(s2dw_wav_abg), allocatable :: wavdyn(:), wavmask(:)
   real(dp), allocatable :: mean(:), var(:), skew(:), kur(:), five(:), six(:)
+  real(dp), allocatable :: mv(:,:,:)
+  character(len=STRING_LEN), allocatable :: desc(:)
+  integer :: nsim, isim, k, nopt
 
   call get_command_argument(1, mode)
 
@@ -146,6 +154,31 @@ program ref_dump
         write(u) wavdyn(jj)%coeff
      end do
      close(u)
+
+  case ('momwrite')
+     call get_command_argument(2, arg); read(arg,*) nsim
+     call get_command_argument(3, arg); read(arg,*) J
+     call get_command_argument(4, arg); read(arg,*) nopt
+     call get_command_argument(5, outf)
+     allocate(mv(0:nsim-1, 0:J, 0:5), desc(0:nsim-1))
+     do isim = 0, nsim-1
+        write(desc(isim), '(a,i0)') 'set', isim
+        do jj = 0, J
+           do k = 0, 5
+              mv(isim, jj, k) = (-1)**k * (isim + 1.5d0) * 10d0**(jj - 3) / (k + 3)
+           end do
+        end do
+     end do
+     select case (nopt)
+     case (0)
+        call s2dw_stat_moments_write(trim(outf), nsim, desc, J, mv(:,:,0), mv(:,:,1), mv(:,:,2), mv(:,:,3))
+     case (1)
+        call s2dw_stat_moments_write(trim(outf), nsim, desc, J, mv(:,:,0), mv(:,:,1), mv(:,:,2), mv(:,:,3), &
+             mv(:,:,4))
+     case default
+        call s2dw_stat_moments_write(trim(outf), nsim, desc, J, mv(:,:,0), mv(:,:,1), mv(:,:,2), mv(:,:,3), &
+             mv(:,:,4), mv(:,:,5))
+     end select
 
   case default
      write(*,'(a)') 'ref_dump: unknown mode'

@@ -4,6 +4,11 @@
 ! Usage: s2dw_hip_stats_test B N alpha mode
 !   mode 0: no mask; 1: a mask of zeros and ones; 2: an all-zero mask on
 !   scale 0 (the routine halts as the reference does).
+! Usage: s2dw_hip_stats_test write nsim J nopt out
+!   s2dw_stat_moments_write into the text file out, for the values
+!   v(isim, jj, k) = (-1)**k * (isim + 1.5) * 10**(jj - 3) / (k + 3)
+!   (k = 0 mean .. 5 six), descriptions "set<isim>", and nopt = 0, 1, 2 of
+!   the optional five, six passed (no GPU work).
 !
 ! The coefficients of scale jj are x(k) = sin(0.37 k + jj) + 0.1 cos(1.3 k)**3
 ! over the column-major index k = 0, 1, ...; the mask is 1 where
@@ -25,8 +30,38 @@ program s2dw_hip_stats_test
   real(dp), allocatable :: mean(:), var(:), skew(:), kur(:), five(:), six(:)
   type(s2dw_wav_abg), allocatable :: wavdyn(:), mask(:)
   logical :: ok
+  real(dp), allocatable :: mv(:,:,:)
+  character(len=STRING_LEN), allocatable :: desc(:)
+  character(len=STRING_LEN) :: outf
+  integer :: nsim, isim, nopt
 
-  call get_command_argument(1, arg); read(arg,*) B
+  call get_command_argument(1, arg)
+  if (trim(arg) == 'write') then
+     call get_command_argument(2, arg); read(arg,*) nsim
+     call get_command_argument(3, arg); read(arg,*) J
+     call get_command_argument(4, arg); read(arg,*) nopt
+     call get_command_argument(5, outf)
+     allocate(mv(0:nsim-1, 0:J, 0:5), desc(0:nsim-1))
+     do isim = 0, nsim-1
+        write(desc(isim), '(a,i0)') 'set', isim
+        do jj = 0, J
+           do k = 0, 5
+              mv(isim, jj, k) = (-1)**k * (isim + 1.5d0) * 10d0**(jj - 3) / (k + 3)
+           end do
+        end do
+     end do
+     if (nopt == 0) then
+        call s2dw_stat_moments_write(trim(outf), nsim, desc, J, mv(:,:,0), mv(:,:,1), mv(:,:,2), mv(:,:,3))
+     else if (nopt == 1) then
+        call s2dw_stat_moments_write(trim(outf), nsim, desc, J, mv(:,:,0), mv(:,:,1), mv(:,:,2), mv(:,:,3), &
+             mv(:,:,4))
+     else
+        call s2dw_stat_moments_write(trim(outf), nsim, desc, J, mv(:,:,0), mv(:,:,1), mv(:,:,2), mv(:,:,3), &
+             mv(:,:,4), mv(:,:,5))
+     end if
+     stop
+  end if
+  read(arg,*) B
   call get_command_argument(2, arg); read(arg,*) N
   call get_command_argument(3, arg); read(arg,*) alpha
   call get_command_argument(4, arg); read(arg,*) mode

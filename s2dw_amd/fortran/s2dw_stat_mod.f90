@@ -8,7 +8,10 @@
 ! (s2dw_hip_stat_moments in include/s2dw_hip.h).  Errors follow the
 ! reference: nj < 2 on a scale halts (S2DW_ERROR_SIZE_INVALID), a variance
 ! below TOL_ZERO reports the non-halting S2DW_ERROR_ARTH_WARNING with zero
-! skewness and higher moments for that scale.
+! skewness and higher moments for that scale.  s2dw_stat_moments_write writes
+! the moments in the reference's text format (one line per set of
+! coefficients, reference 315-367).  The reference's s2dw_stat_histogram is
+! not provided (no program calls it; its last bin reads past hist_bins).
 !------------------------------------------------------------------------------
 module s2dw_stat_mod
 
@@ -21,7 +24,9 @@ module s2dw_stat_mod
 
   private
 
-  public :: s2dw_stat_moments
+  public :: &
+       s2dw_stat_moments, &
+       s2dw_stat_moments_write
 
   interface
      function c_stat_moments(ptrs, J, B, N, alpha, mean, var, skew, kur, mptrs, five, six) &
@@ -84,5 +89,45 @@ contains
        call s2dw_error(int(code), 's2dw_stat_moments')
     end select
   end subroutine s2dw_stat_moments
+
+  ! s2dw_stat_moments_write, reference 315-367: per set isim one record
+  ! "description, J, " followed by mu(isim,0:J), var, skew, kur and, when
+  ! present, five and six, each value as e27.20 followed by ", ".
+  subroutine s2dw_stat_moments_write(filename, nsim, description, J, mu, var, skew, kur, five, six)
+    character(len=*), intent(in) :: filename
+    integer, intent(in) :: nsim
+    character(len=STRING_LEN), intent(in) :: description(0:nsim-1)
+    integer, intent(in) :: J
+    real(dp), intent(in) :: mu(0:nsim-1,0:J)
+    real(dp), intent(in) :: var(0:nsim-1,0:J)
+    real(dp), intent(in) :: skew(0:nsim-1,0:J)
+    real(dp), intent(in) :: kur(0:nsim-1,0:J)
+    real(dp), optional :: five(0:nsim-1,0:J)
+    real(dp), optional :: six(0:nsim-1,0:J)
+    integer :: u, isim
+    open(newunit=u, file=filename, form='formatted', status='replace', action='write')
+    do isim = 0, nsim-1
+       write(u, '(a,a)', advance='no') trim(description(isim)), ', '
+       write(u, '(i4,a)', advance='no') J, ', '
+       call put(mu(isim,:))
+       call put(var(isim,:))
+       call put(skew(isim,:))
+       call put(kur(isim,:))
+       if (present(five)) then
+          call put(five(isim,:))
+          if (present(six)) call put(six(isim,:))
+       end if
+       write(u, *)
+    end do
+    close(u)
+  contains
+    subroutine put(v)
+      real(dp), intent(in) :: v(0:J)
+      integer :: jj
+      do jj = 0, J
+         write(u, '(e27.20,a)', advance='no') v(jj), ', '
+      end do
+    end subroutine put
+  end subroutine s2dw_stat_moments_write
 
 end module s2dw_stat_mod

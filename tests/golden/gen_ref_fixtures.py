@@ -97,6 +97,10 @@ def moments_fixtures():
         else:
             out[k + "x_after"] = after
     np.savez_compressed(os.path.join(HERE, "ref_moments.npz"), **out)
+    # s2dw_stat_moments_write: the reference's text format, byte for byte
+    for nopt in (0, 1, 2):
+        with open(os.path.join(HERE, f"ref_moments_write_{nopt}.txt"), "wb") as fh:
+            fh.write(run("momwrite", 3, 4, nopt))
 
 
 def main():

@@ -42,6 +42,20 @@ def test_modules_build():
     assert os.path.exists(HIP_TEST) and os.path.exists(STATS_TEST)
 
 
+@pytest.mark.skipif(shutil.which("amdflang") is None, reason="no Fortran compiler")
+@pytest.mark.parametrize("nopt", [0, 1, 2])
+def test_stat_moments_write_matches_reference_bytes(tmp_path, nopt):
+    # the reference's s2dw_stat_moments_write (stat_mod.f90:315-367) wrote the
+    # fixture from the same values; no GPU work is involved
+    subprocess.run(["make", "-C", FDIR], check=True, capture_output=True)
+    out = tmp_path / "m.txt"
+    r = subprocess.run([STATS_TEST, "write", "3", "4", str(nopt), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    want = open(os.path.join(ROOT, "tests", "golden", f"ref_moments_write_{nopt}.txt"), "rb").read()
+    assert out.read_bytes() == want
+
+
 def test_stat_module_interface_matches_reference():
     # s2dw_stat_moments keeps the reference's dummy arguments
     # (src/mod/s2dw_stat_mod.f90:81-94)
