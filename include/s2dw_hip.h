@@ -229,6 +229,22 @@ int s2dw_hip_stat_moments(double *const *wavdyn, int J, int B, int N, double alp
  * 6 nj, 7 flag (0 ok, 1 variance below TOL_ZERO, 2 nj < 2: values NaN). */
 int s2dw_hip_stat_moments_device(s2dw_hip_plan_t plan, double *d_wav, const double *d_mask, double *d_out);
 
+/* Analysis followed by the moments, without storing the wavelet coefficients:
+ * the analysis of s2dw_core_analysis_flm2wav_dynamic (s2dw_core_mod.f90:
+ * 572-775) whose alpha-transform stage reduces each workgroup's coefficients
+ * to a moments record instead of writing them, then the per-scale moments of
+ * s2dw_stat_moments without a mask (s2dw_stat_mod.f90:81-275), as the
+ * s2dw_wstats pipeline computes them (s2dw_wstats.f90:181-190).  d_out as in
+ * s2dw_hip_stat_moments_device, d_scoeff and d_flm as in
+ * s2dw_hip_analysis_device.  Enqueued on the plan's stream (graph replay
+ * when enabled), no allocation.  Returns S2DW_ERROR_ARG_INVALID when
+ * s2dw_hip_plan_moments_fusable is 0 for the plan. */
+int s2dw_hip_analysis_moments_device(s2dw_hip_plan_t plan, double *d_out, double *d_scoeff, const double *d_flm);
+/* 1 when every non-empty scale of the plan runs in the grouped or paired
+ * FFT launches (all configurations with B <= 1024 on the workgroup engine),
+ * which s2dw_hip_analysis_moments_device needs; 0 otherwise. */
+int s2dw_hip_plan_moments_fusable(s2dw_hip_plan_t plan);
+
 /* ---------------------------------------------------------------------
  * Request loop (no counterpart in the reference, whose routines run one call
  * at a time from one host thread, s2dw_core_mod.f90).  A server owns a worker

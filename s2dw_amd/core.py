@@ -119,6 +119,8 @@ def lib() -> C.CDLL:
         L.s2dw_hip_stat_moments.argtypes = [C.POINTER(_dp), C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
                                             _dp, C.POINTER(_dp), _dp, _dp]
         L.s2dw_hip_stat_moments_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.s2dw_hip_analysis_moments_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.s2dw_hip_plan_moments_fusable.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -389,6 +391,25 @@ class Plan:
 
     STAGES = ["ana_relayout", "ana_contract", "ana_beta_fft", "ana_alpha_c2r", "ana_scoeff", "syn_alpha_r2c",
               "syn_beta_fft", "syn_contract", "syn_assemble", "ana_small_ffts", "syn_small_ffts"]
+
+    @property
+    def moments_fusable(self) -> bool:
+        """Whether analysis_moments can run on this plan."""
+        return bool(lib().s2dw_hip_plan_moments_fusable(self.h))
+
+    def analysis_moments(self, d_flm, d_scoeff, d_out=None):
+        """Enqueue an analysis that reduces the wavelet coefficients to their
+        per-scale moments without storing them; returns d_out ((8, J+1)
+        float64, rows MOMENT_ROWS) and fills d_scoeff."""
+        import torch
+
+        if d_out is None:
+            d_out = torch.empty((8, self.J + 1), dtype=torch.float64, device=d_flm.device)
+        _check(lib().s2dw_hip_analysis_moments_device(self.h, C.c_void_p(d_out.data_ptr()),
+                                                      C.c_void_p(d_scoeff.data_ptr()),
+                                                      C.c_void_p(d_flm.data_ptr())),
+               "s2dw_hip_analysis_moments_device")
+        return d_out
 
     def set_timing(self, enable: bool):
         _check(lib().s2dw_hip_plan_set_timing(self.h, int(bool(enable))), "s2dw_hip_plan_set_timing")

@@ -375,6 +375,18 @@ int s2dw_hip_stat_moments_device(s2dw_hip_plan_t plan, double *d_wav, const doub
   return moments_run(p.stream, off.data(), cnt.data(), n, d_wav, d_mask, p.d_mom, d_out);
 }
 
+int s2dw_hip_analysis_moments_device(s2dw_hip_plan_t plan, double *d_out, double *d_scoeff, const double *d_flm) {
+  if (!plan || !d_out || !d_scoeff || !d_flm) return kErrArgInvalid;
+  if (!plan->p.momf_ok) {
+    set_error("s2dw_hip_analysis_moments_device: this plan runs a scale outside the grouped launches; use "
+              "s2dw_hip_analysis_device and s2dw_hip_stat_moments_device");
+    return kErrArgInvalid;
+  }
+  return run_graphed(&plan->p, 2, d_out, d_scoeff, d_flm);
+}
+
+int s2dw_hip_plan_moments_fusable(s2dw_hip_plan_t plan) { return plan && plan->p.momf_ok ? 1 : 0; }
+
 namespace {
 struct DevBufs {
   std::vector<void *> v;

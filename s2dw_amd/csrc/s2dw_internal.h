@@ -170,8 +170,15 @@ struct Plan {
   void *groups = nullptr;  // grouped launches of the small scales (kernels' own state)
   bool graphs = false;
   hipStream_t cap = nullptr;
-  double *d_mom = nullptr;    // moments partials (allocated at first use)
+  double *d_mom = nullptr;    // moments partials of s2dw_hip_stat_moments_device
   long long mom_doubles = 0;
+  // analysis reducing to moments (s2dw_hip_analysis_moments_device): one
+  // record per alpha-c2r workgroup, scale j owning records momf_rec0[j] ..
+  // momf_rec0[j+1]-1; available when every non-empty scale runs in the
+  // grouped or paired launches
+  bool momf_ok = false;
+  std::vector<long long> momf_rec0;
+  double *d_momf = nullptr;
   std::vector<GraphEntry> gcache;
   unsigned long long gtick = 0;
 };
@@ -179,9 +186,13 @@ struct Plan {
 int plan_build(Plan *p, int B, int N, int J, double alpha, const double *K, const double *Phi2, const double *S,
                int precision, hipStream_t stream, bool own_stream);
 void plan_free(Plan *p);
-int run_analysis(Plan *p, double *d_wav, double2 *d_scoeff, const double2 *d_flm);
+// d_mom != NULL: the alpha-c2r stage reduces to moments records instead of
+// writing d_wav (which may be NULL), and d_mom receives the per-scale moments
+// (rows as moments_run's d_out)
+int run_analysis(Plan *p, double *d_wav, double2 *d_scoeff, const double2 *d_flm, double *d_mom = nullptr);
 int run_synthesis(Plan *p, double2 *d_flm, const double *d_wav, const double2 *d_scoeff);
-// run_analysis (kind 0) or run_synthesis (kind 1) through the plan's graph
+// run_analysis (kind 0; kind 2: reducing to moments, out0 = d_mom) or
+// run_synthesis (kind 1) through the plan's graph
 // cache when graphs are on and stage timing is off, eagerly otherwise
 int run_graphed(Plan *p, int kind, void *out0, void *out1, const void *in);
 int plan_autotune(Plan *p, int reps, std::string *report);
@@ -195,6 +206,10 @@ int pack_static(Plan *p, const double *d_packed, double *d_static, bool to_stati
 long long moments_partial_doubles(const long long *count, int nscales);
 int moments_run(hipStream_t st, const long long *off, const long long *count, int nscales, double *d_wav,
                 const double *d_mask, double *d_part, double *d_out);
+// Final merge of moments records already in d_part: scale s (count[s]
+// values) owns records rec0[s] .. rec0[s+1]-1 (8 doubles each).
+int moments_finish(hipStream_t st, const long long *count, const long long *rec0, int nscales, const double *d_part,
+                   double *d_out);
 int wigner_plane(int l, double *out, double *out2);
 
 }  // namespace s2dw

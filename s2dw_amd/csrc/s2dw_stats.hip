@@ -21,6 +21,7 @@
 #include <cmath>
 
 #include "s2dw_internal.h"
+#include "s2dw_moments.h"
 
 namespace s2dw {
 namespace {
@@ -29,8 +30,7 @@ constexpr int kMomThreads = 256;
 constexpr int kMomPairs = 16;  // double2 loads per lane: 32 values
 constexpr long long kMomChunk = (long long)kMomThreads * kMomPairs * 2;
 constexpr int kMomBatch = 32;  // scales per launch
-// TOL_ZERO of s2dw_types_mod.f90:51, a default-real literal
-constexpr double kTolZero = (double)1e-12f;
+constexpr double kTolZero = kMomTolZero;
 
 struct MomTab {
   long long off[kMomBatch];    // scale offset in the packed coefficients (doubles)
@@ -132,35 +132,10 @@ __device__ __forceinline__ Mom mom_block_reduce(Mom m, Mom *sh) {
   return m;
 }
 
-// Sum of K values over the workgroup, returned to every lane in the same
-// bits (xor butterflies in the wave, then the waves in a fixed order).
-template <int K>
-__device__ __forceinline__ void block_sum(double (&v)[K], double (*sh)[K]) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1)
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], o);
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int k = 0; k < K; ++k) sh[wave][k] = v[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double t = sh[0][k];
-#pragma unroll
-    for (int w = 1; w < kMomThreads / 64; ++w) t += sh[w][k];
-    v[k] = t;
-  }
-  __syncthreads();
-}
-
 template <bool MASKED>
 __global__ __launch_bounds__(kMomThreads) void k_mom_partial(const MomTab t, double *__restrict__ wav,
                                                              const double *__restrict__ mask,
                                                              double *__restrict__ part, int part_base) {
-  __shared__ double sh3[kMomThreads / 64][3];
-  __shared__ double sh5[kMomThreads / 64][5];
   const int b = blockIdx.x;
   int s = 0;
   while (s + 1 < t.nscale && b >= t.blk0[s + 1]) ++s;
@@ -183,7 +158,7 @@ __global__ __launch_bounds__(kMomThreads) void k_mom_partial(const MomTab t, dou
   // the masked coefficients replace the input (s2dw_stat_mod.f90:136-140);
   // values outside the mask (or past the end) take part with weight 0
   double v[2 * kMomPairs], wt[2 * kMomPairs];
-  double c3[3] = {0, 0, 0};  // count, sum, sum of mask values
+  double ms = 0;
 #pragma unroll
   for (int e = 0; e < kMomPairs; ++e) {
     const long long i = base + 2LL * (e * kMomThreads + threadIdx.x);
@@ -194,37 +169,9 @@ __global__ __launch_bounds__(kMomThreads) void k_mom_partial(const MomTab t, dou
     v[2 * e + 1] = y.y;
     wt[2 * e] = in && fabs(mv[e].x - 1.0) < kTolZero ? 1.0 : 0.0;
     wt[2 * e + 1] = in && fabs(mv[e].y - 1.0) < kTolZero ? 1.0 : 0.0;
-    if (in) c3[2] += mv[e].x + mv[e].y;
+    if (in) ms += mv[e].x + mv[e].y;
   }
-#pragma unroll
-  for (int e = 0; e < 2 * kMomPairs; ++e) {
-    c3[0] += wt[e];
-    c3[1] += wt[e] * v[e];
-  }
-  block_sum<3>(c3, sh3);
-  const double mu = c3[0] > 0 ? c3[1] / c3[0] : 0.0;
-  double p5[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-  for (int e = 0; e < 2 * kMomPairs; ++e) {
-    const double s1 = wt[e] * (v[e] - mu), s2 = s1 * s1, s3 = s2 * s1;
-    p5[0] += s2;
-    p5[1] += s3;
-    p5[2] += s2 * s2;
-    p5[3] += s3 * s2;
-    p5[4] += s3 * s3;
-  }
-  block_sum<5>(p5, sh5);
-  if (threadIdx.x == 0) {
-    double *o = part + 8LL * (part_base + b);
-    o[0] = c3[0];
-    o[1] = mu;
-    o[2] = p5[0];
-    o[3] = p5[1];
-    o[4] = p5[2];
-    o[5] = p5[3];
-    o[6] = p5[4];
-    o[7] = c3[2];
-  }
+  mom_record<2 * kMomPairs, kMomThreads>(v, wt, ms, part + 8LL * (part_base + b));
 }
 
 // One workgroup per scale: merge its partials, then the reference's formulas.
@@ -334,6 +281,24 @@ int moments_run(hipStream_t st, const long long *off, const long long *count, in
                        nscales, s0);
     S2DW_HIP_CHECK(hipGetLastError());
     part_base += nb;
+  }
+  return kErrNone;
+}
+
+int moments_finish(hipStream_t st, const long long *count, const long long *rec0, int nscales, const double *d_part,
+                   double *d_out) {
+  if (nscales <= 0 || !d_part || !d_out) return kErrArgInvalid;
+  for (int s0 = 0; s0 < nscales; s0 += kMomBatch) {
+    MomTab t{};
+    t.nscale = nscales - s0 < kMomBatch ? nscales - s0 : kMomBatch;
+    t.masked = 0;
+    for (int s = 0; s <= t.nscale; ++s) {
+      if (s < t.nscale) t.count[s] = count[s0 + s];
+      t.blk0[s] = (int)(rec0[s0 + s] - rec0[s0]);
+    }
+    hipLaunchKernelGGL(k_mom_final, dim3(t.nscale), dim3(kMomThreads), 0, st, t, d_part, (int)rec0[s0], d_out,
+                       nscales, s0);
+    S2DW_HIP_CHECK(hipGetLastError());
   }
   return kErrNone;
 }

@@ -167,3 +167,53 @@ def test_device_moments_flags_and_determinism():
     assert out[7, 1] == 2 and out[6, 1] == 0 and np.all(np.isnan(out[:6, 1]))
     assert np.all(out[7, 2:] == 0)
     plan.close()
+
+
+@pytest.mark.parametrize("B,N,alpha,J", [(64, 3, 2.0, None), (81, 5, 3.0, None), (27, 4, 3.0, None),
+                                         (128, 8, 2.0, 5), (40, 5, 2.0, 3), (512, 8, 2.0, 7)])
+def test_analysis_moments_without_storing(B, N, alpha, J):
+    """The analysis that reduces to moments in its alpha stage against the
+    moments of the stored coefficients (same transform, different chunking
+    of the sums) and against pairwise sums over them."""
+    import torch
+
+    say(f"analysis reducing to moments B={B} N={N} alpha={alpha} J={J}")
+    st, plan, d_w = device_analysis(B, N, alpha, J, 60 + B)
+    assert plan.moments_fusable
+    wavs = host_scales(plan, d_w)
+    d_ref = plan.stat_moments(d_w)
+    f = seeded_flm(B, 60 + B, red=True)
+    d_f = torch.from_numpy(np.ascontiguousarray(f.T)).cuda()
+    d_sc, d_sc2 = plan.empty_scoeff(), plan.empty_scoeff()
+    plan.analysis(d_f, plan.empty_wav(), d_sc)
+    d_out = plan.analysis_moments(d_f, d_sc2)
+    plan.sync()
+    ref, out = d_ref.cpu().numpy(), d_out.cpu().numpy()
+    assert torch.equal(d_sc, d_sc2)
+    np.testing.assert_array_equal(out[6:], ref[6:])  # nj and flags
+    assert_moments(list(out[:6]), list(ref[:6]), "fused vs stored")
+    assert_moments(list(out[:6]), list(pairwise_moments(wavs)), "fused vs pairwise")
+    plan.close()
+
+
+def test_analysis_moments_graph_replay():
+    import torch
+
+    B, N, alpha = 64, 5, 2.0
+    say("analysis reducing to moments, graph replay B=64 N=5")
+    st = O.Setup(B, N, alpha)
+    plan = core.Plan(B, N, st.J, alpha, st.K, st.Phi2, st.S)
+    d_f = torch.from_numpy(np.ascontiguousarray(seeded_flm(B, 5, red=True).T)).cuda()
+    d_sc = plan.empty_scoeff()
+    eager = plan.analysis_moments(d_f, d_sc).clone()
+    plan.sync()
+    plan.set_graphs(True)
+    d_out = torch.empty_like(eager)
+    for r in range(3):
+        d_out.fill_(-1.0)
+        plan.analysis_moments(d_f, d_sc, d_out)
+        plan.sync()
+        assert torch.equal(d_out, eager), r
+        torch.cuda.synchronize()
+        _ = (d_f * 2).sum().item()  # eager work between replays
+    plan.close()
