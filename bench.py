@@ -156,7 +156,7 @@ def traffic_from(fetch_path, write_path):
         for name, vals in per.items():
             if "s2dw" not in name:
                 continue
-            key = name.split("(")[0].replace("void ", "").replace("s2dw::", "")
+            key = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").replace("s2dw::", "")
             d = out.setdefault(key, {})
             d[counter.lower() + "_bytes"] = scale * sum(vals) / len(vals)
             d[counter.lower() + "_scale"] = scale
