@@ -4,15 +4,17 @@
 // with its optional mask.
 //
 // One pass over HBM.  Each workgroup reduces an 8192-value chunk of one scale
-// to (count, mean, central sums M_2..M_6, sum of mask values) by an exact
-// two-pass over the values its lanes hold in registers: a workgroup sum gives
-// the chunk mean, a second workgroup sum the powers of the deviations from
-// it.  A second kernel merges the chunk results of each scale in a fixed
-// order (the result does not depend on scheduling) with the update formula
-// for central sums of P. Pebay, "Formulas for robust, one-pass parallel
+// to a record (count, mean, central sums M_2..M_6, sum of mask values) by an
+// exact two-pass over the values its lanes hold in registers: a workgroup sum
+// gives the chunk mean, a second workgroup sum the powers of the deviations
+// from it.  A second kernel combines the records of each scale the same way:
+// the scale mean from the records' counts and means, then each record's
+// central sums moved to that mean with the binomial theorem and summed (the
+// pairwise update of P. Pebay, "Formulas for robust, one-pass parallel
 // computation of covariances and arbitrary-order statistical moments",
-// Sandia report SAND2008-6212 (2008), eq. 3.1, and applies the reference's
-// finishing formulas: nj = nint(sum of mask),
+// Sandia report SAND2008-6212 (2008), written for many parts at once), in a
+// fixed order, so the result does not depend on scheduling.  It then applies
+// the reference's finishing formulas: nj = nint(sum of mask),
 // mean = (sum over included values) / nj, central sums re-centred on that
 // mean, var = (sum s^2 - (sum s)^2 / nj) / (nj - 1), and zero higher moments
 // when var < TOL_ZERO (s2dw_stat_mod.f90:143-150, 229-268).
@@ -39,98 +41,6 @@ struct MomTab {
   int nscale;
   int masked;
 };
-
-// count, mean, M_2..M_6 about the mean, sum of mask values
-struct Mom {
-  double n, mu, m2, m3, m4, m5, m6, ms;
-};
-
-__device__ __forceinline__ Mom mom_zero() { return Mom{0, 0, 0, 0, 0, 0, 0, 0}; }
-
-// a <- a merged with b (Pebay 2008, eq. 3.1, orders 2..6):
-//   M_p = A_p + B_p + sum_{k=1}^{p-2} C(p,k) d^k [(-nb/n)^k A_{p-k} + (na/n)^k B_{p-k}]
-//         + q [(d na/n)^{p-1} - (-d nb/n)^{p-1}],   d = mu_b - mu_a, q = na nb d / n.
-__device__ __forceinline__ void mom_merge(Mom &a, const Mom &b) {
-  if (b.n == 0) {
-    a.ms += b.ms;
-    return;
-  }
-  if (a.n == 0) {
-    const double ms = a.ms + b.ms;
-    a = b;
-    a.ms = ms;
-    return;
-  }
-  const double n = a.n + b.n, d = b.mu - a.mu, ra = a.n / n, rb = b.n / n;
-  const double q = a.n * rb * d;
-  const double A[7] = {0, 0, a.m2, a.m3, a.m4, a.m5, a.m6};
-  const double Bm[7] = {0, 0, b.m2, b.m3, b.m4, b.m5, b.m6};
-  double dk[6], pa[6], pb[6], ea[6], eb[6];  // d^k, (-rb)^k, ra^k, (ra d)^k, (-rb d)^k
-  dk[0] = pa[0] = pb[0] = ea[0] = eb[0] = 1.0;
-#pragma unroll
-  for (int k = 1; k < 6; ++k) {
-    dk[k] = dk[k - 1] * d;
-    pa[k] = pa[k - 1] * -rb;
-    pb[k] = pb[k - 1] * ra;
-    ea[k] = ea[k - 1] * (ra * d);
-    eb[k] = eb[k - 1] * (-rb * d);
-  }
-  constexpr double C[7][5] = {{0}, {0}, {0}, {0, 3}, {0, 4, 6}, {0, 5, 10, 10}, {0, 6, 15, 20, 15}};
-  double M[7];
-#pragma unroll
-  for (int p = 2; p <= 6; ++p) {
-    double s = A[p] + Bm[p];
-#pragma unroll
-    for (int k = 1; k <= p - 2; ++k) s += C[p][k] * dk[k] * (pa[k] * A[p - k] + pb[k] * Bm[p - k]);
-    M[p] = s + q * (ea[p - 1] - eb[p - 1]);
-  }
-  a.n = n;
-  a.mu += d * rb;
-  a.m2 = M[2];
-  a.m3 = M[3];
-  a.m4 = M[4];
-  a.m5 = M[5];
-  a.m6 = M[6];
-  a.ms += b.ms;
-}
-
-__device__ __forceinline__ Mom mom_shfl_xor(const Mom &a, int lane_mask) {
-  Mom b;
-  b.n = __shfl_xor(a.n, lane_mask);
-  b.mu = __shfl_xor(a.mu, lane_mask);
-  b.m2 = __shfl_xor(a.m2, lane_mask);
-  b.m3 = __shfl_xor(a.m3, lane_mask);
-  b.m4 = __shfl_xor(a.m4, lane_mask);
-  b.m5 = __shfl_xor(a.m5, lane_mask);
-  b.m6 = __shfl_xor(a.m6, lane_mask);
-  b.ms = __shfl_xor(a.ms, lane_mask);
-  return b;
-}
-
-// Merge the Mom of every lane of the workgroup; lane 0 of wave 0 returns
-// the result.  The pairing is fixed (xor tree in the wave, waves in order).
-__device__ __forceinline__ Mom mom_block_reduce(Mom m, Mom *sh) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const Mom o2 = mom_shfl_xor(m, o);
-    // both partners compute the same merge in the same operand order
-    if (lane & o) {
-      Mom t = o2;
-      mom_merge(t, m);
-      m = t;
-    } else {
-      mom_merge(m, o2);
-    }
-  }
-  if (lane == 0) sh[wave] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    m = sh[0];
-    for (int w = 1; w < kMomThreads / 64; ++w) mom_merge(m, sh[w]);
-  }
-  return m;
-}
 
 template <bool MASKED>
 __global__ __launch_bounds__(kMomThreads) void k_mom_partial(const MomTab t, double *__restrict__ wav,
@@ -174,40 +84,58 @@ __global__ __launch_bounds__(kMomThreads) void k_mom_partial(const MomTab t, dou
   mom_record<2 * kMomPairs, kMomThreads>(v, wt, ms, part + 8LL * (part_base + b));
 }
 
-// One workgroup per scale: merge its partials, then the reference's formulas.
-// out[k * nout + jj], k = 0 mean, 1 var, 2 skew, 3 kur, 4 five, 5 six, 6 nj,
-// 7 flag (0 ok, 1 var < TOL_ZERO: higher moments set to zero, 2 nj < 2).
+// One workgroup per scale: combine its records, then the reference's
+// formulas.  out[k * nout + jj], k = 0 mean, 1 var, 2 skew, 3 kur, 4 five,
+// 5 six, 6 nj, 7 flag (0 ok, 1 var < TOL_ZERO: higher moments set to zero,
+// 2 nj < 2).
 __global__ __launch_bounds__(kMomThreads) void k_mom_final(const MomTab t, const double *__restrict__ part,
                                                            int part_base, double *__restrict__ out, int nout,
                                                            int jj0) {
-  __shared__ Mom sh[kMomThreads / 64];
+  __shared__ double sh3[kMomThreads / 64][3];
+  __shared__ double sh5[kMomThreads / 64][5];
   const int s = blockIdx.x;
   const int p0 = t.blk0[s], p1 = t.blk0[s + 1];
-  Mom m = mom_zero();
+  // records (n_i, mu_i, M_{2..6,i}, ms_i): the scale's count and mean from
+  // sum n_i and sum n_i mu_i, then each record's central sums moved to that
+  // mean, T_p = sum_k C(p,k) (mu_i - mu)^(p-k) M_{k,i} with M_0 = n_i, M_1 = 0,
+  // and summed
+  double c3[3] = {0, 0, 0};
   for (int p = p0 + (int)threadIdx.x; p < p1; p += kMomThreads) {
     const double *o = part + 8LL * (part_base + p);
-    const Mom b = Mom{o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7]};
-    mom_merge(m, b);
+    c3[0] += o[0];
+    c3[1] += o[0] * o[1];
+    c3[2] += o[7];
   }
-  m = mom_block_reduce(m, sh);
+  block_sum<3, kMomThreads>(c3, sh3);
+  const double c = c3[0], mu = c > 0 ? c3[1] / c : 0.0;
+  double T[5] = {0, 0, 0, 0, 0};  // T_2..T_6 about mu
+  for (int p = p0 + (int)threadIdx.x; p < p1; p += kMomThreads) {
+    const double *o = part + 8LL * (part_base + p);
+    const double n = o[0], d = o[1] - mu, M2 = o[2], M3 = o[3], M4 = o[4], M5 = o[5];
+    const double d2 = d * d, d3 = d2 * d, d4 = d2 * d2;
+    T[0] += M2 + n * d2;
+    T[1] += M3 + 3 * d * M2 + n * d3;
+    T[2] += M4 + 4 * d * M3 + 6 * d2 * M2 + n * d4;
+    T[3] += M5 + 5 * d * M4 + 10 * d2 * M3 + 10 * d3 * M2 + n * d4 * d;
+    T[4] += o[6] + 6 * d * M5 + 15 * d2 * M4 + 20 * d3 * M3 + 15 * d4 * M2 + n * d3 * d3;
+  }
+  block_sum<5, kMomThreads>(T, sh5);
   if (threadIdx.x != 0) return;
-  const double c = m.n;
   // nj = nint(sum(mask)) with a mask, the number of values without
   // (s2dw_stat_mod.f90:143-148); Fortran nint rounds halves away from zero
-  const double nj = t.masked ? round(m.ms) : (double)t.count[s];
+  const double nj = t.masked ? round(c3[2]) : (double)t.count[s];
   double r[8];
   if (nj < 2) {
     for (int k = 0; k < 6; ++k) r[k] = NAN;
     r[6] = nj;
     r[7] = 2;
   } else {
-    const double mean = c * m.mu / nj;
+    const double mean = c * mu / nj;
     // sums of s = x - mean over the included values, from the central sums
-    // about their own mean mu: T_p = sum_k C(p,k) (mu - mean)^(p-k) M_k,
-    // M_0 = c, M_1 = 0
-    const double d = m.mu - mean;
-    const double Mk[7] = {c, 0, m.m2, m.m3, m.m4, m.m5, m.m6};
-    double T[7];
+    // about mu: S_p = sum_k C(p,k) (mu - mean)^(p-k) T_k, T_0 = c, T_1 = 0
+    const double d = mu - mean;
+    const double Tk[7] = {c, 0, T[0], T[1], T[2], T[3], T[4]};
+    double S[7];
     constexpr double Cb[7][7] = {{1},
                                  {1, 1},
                                  {1, 2, 1},
@@ -218,13 +146,13 @@ __global__ __launch_bounds__(kMomThreads) void k_mom_final(const MomTab t, const
     for (int p = 1; p <= 6; ++p) {
       double acc = 0, dp = 1;  // d^(p-k), k from p down to 0
       for (int k = p; k >= 0; --k) {
-        acc += Cb[p][k] * dp * Mk[k];
+        acc += Cb[p][k] * dp * Tk[k];
         dp *= d;
       }
-      T[p] = acc;
+      S[p] = acc;
     }
-    const double ep = T[1];
-    const double var = (T[2] - ep * ep / nj) / (nj - 1);
+    const double ep = S[1];
+    const double var = (S[2] - ep * ep / nj) / (nj - 1);
     r[0] = mean;
     r[1] = var;
     r[6] = nj;
@@ -233,10 +161,10 @@ __global__ __launch_bounds__(kMomThreads) void k_mom_final(const MomTab t, const
       r[7] = 1;
     } else {
       const double sd = sqrt(var);
-      r[2] = T[3] / (nj * sd * sd * sd);
-      r[3] = T[4] / (nj * var * var) - 3.0;
-      r[4] = T[5] / (nj * sd * sd * sd * sd * sd);
-      r[5] = T[6] / (nj * var * var * var) - 15.0;
+      r[2] = S[3] / (nj * sd * sd * sd);
+      r[3] = S[4] / (nj * var * var) - 3.0;
+      r[4] = S[5] / (nj * sd * sd * sd * sd * sd);
+      r[5] = S[6] / (nj * var * var * var) - 15.0;
       r[7] = 0;
     }
   }
