@@ -193,7 +193,9 @@ def _mix_requests(rng, count, Ls, Ns, alphas, log_uniform=False):
 def bench_batch(args):
     """64 transforms (32 analyses, 32 syntheses of their resident outputs),
     B from {64,128,256,512}, N from {1,3,5,8}, alpha = 2, J = J_max, all
-    submitted at once to the request loop; plans warmed by a first pass."""
+    submitted at once to the request loop.  Two warm passes (the first builds
+    the plans, the second still grows the pinned staging pool), then the
+    median of three timed passes."""
     from s2dw_amd.serve import Server
 
     rng = np.random.default_rng(4)
@@ -210,11 +212,14 @@ def bench_batch(args):
             for t in ta:
                 srv.release(t)
             return t1 - t0, err, [t.latency_ms for t in ta + ts]
-        one_pass()  # builds the plans
-        wall, err, lat = one_pass()
+        warm = [one_pass()[0] for _ in range(2)]
+        runs = sorted((one_pass() for _ in range(3)), key=lambda r: r[0])
+        wall, err, lat = runs[1]
+        err = max(r[1] for r in runs)
     out = {"workload": "batch of 64 transforms (32 analysis + 32 synthesis), L in {64,128,256,512}, "
                        "N in {1,3,5,8}, alpha=2, J=J_max, seeded f_lm",
            "transforms_per_s": 64 / wall, "wall_ms": 1e3 * wall, "max_round_trip_err": err,
+           "timed_passes_ms": [1e3 * r[0] for r in runs], "warm_passes_ms": [1e3 * w for w in warm],
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
            "dtype": "f64", "n_gpus": 1}
     print(json.dumps(out), flush=True)
