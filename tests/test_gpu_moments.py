@@ -217,3 +217,28 @@ def test_analysis_moments_graph_replay():
         torch.cuda.synchronize()
         _ = (d_f * 2).sum().item()  # eager work between replays
     plan.close()
+
+
+def test_host_moments_many_scales():
+    """alpha = 1.1 at B = 64: 45 scales, more than one launch batch of the
+    moments kernels (32 scales), with a mask."""
+    B, N, alpha = 64, 2, 1.1
+    J = O.comp_Jmax(B, alpha)
+    assert J + 1 > 32
+    say(f"moments with {J + 1} scales")
+    his, _ = O.band_limits(J, B, alpha)
+    rng = np.random.default_rng(45)
+    wavs = [np.asfortranarray(rng.standard_normal((2 * h - 1, 2 * h, N)) * (1 + j)) for j, h in enumerate(his)]
+    masks = [np.asfortranarray((rng.random(w.shape) < 0.8).astype(np.float64)) for w in wavs]
+    got = core.s2dw_stat_moments([w.copy(order="F") for w in wavs], J, B, N, alpha)
+    code, *want = O.stat_moments([w.copy(order="F") for w in wavs], J, B, N, alpha)
+    assert code == 0
+    assert_moments(got, want, "many scales")
+    wg = [w.copy(order="F") for w in wavs]
+    got = core.s2dw_stat_moments(wg, J, B, N, alpha, masks)
+    wo = [w.copy(order="F") for w in wavs]
+    code, *want = O.stat_moments(wo, J, B, N, alpha, masks)
+    assert code == 0
+    assert_moments(got, want, "many scales, masked")
+    for a, b in zip(wg, wo):
+        np.testing.assert_array_equal(a, b)
