@@ -390,7 +390,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(requests shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+              f"(independent transforms shard across GPUs with no communication: run one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
