@@ -242,3 +242,41 @@ def test_host_moments_many_scales():
     assert_moments(got, want, "many scales, masked")
     for a, b in zip(wg, wo):
         np.testing.assert_array_equal(a, b)
+
+
+def test_analysis_moments_refused_when_not_fusable():
+    """A plan whose FFTs run outside the grouped launches (here the wave FFT
+    engine, chosen by S2DW_HIP_FFT at plan creation) cannot reduce the
+    analysis to moments; the entry point says so instead of computing
+    something else, and the unfused path still works on that plan."""
+    import os
+
+    import torch
+
+    B, N, alpha = 64, 3, 2.0
+    say("moments fusion refused on a wave-engine plan")
+    st = O.Setup(B, N, alpha)
+    old = os.environ.get("S2DW_HIP_FFT")
+    os.environ["S2DW_HIP_FFT"] = "wave"
+    try:
+        plan = core.Plan(B, N, st.J, alpha, st.K, st.Phi2, st.S)
+    finally:
+        if old is None:
+            del os.environ["S2DW_HIP_FFT"]
+        else:
+            os.environ["S2DW_HIP_FFT"] = old
+    assert not plan.moments_fusable
+    f = seeded_flm(B, 9, red=True)
+    d_f = torch.from_numpy(np.ascontiguousarray(f.T)).cuda()
+    d_sc = plan.empty_scoeff()
+    with pytest.raises(core.S2dwError) as e:
+        plan.analysis_moments(d_f, d_sc)
+    assert e.value.code == core.S2DW_ERROR_ARG_INVALID
+    d_w = plan.empty_wav()
+    plan.analysis(d_f, d_w, d_sc)
+    d_out = plan.stat_moments(d_w)
+    plan.sync()
+    code, *want = O.stat_moments(host_scales(plan, d_w), plan.J, B, N, alpha)
+    assert code == 0
+    assert_moments(list(d_out.cpu().numpy()[:6]), want, "wave engine")
+    plan.close()
