@@ -262,6 +262,45 @@ def test_autotuned_plan_matches_default(B, N):
     plan.close()
 
 
+@pytest.mark.parametrize("B", [1000, 1024])
+def test_parity_largest_band_limits(B):
+    """The largest band limits a plan accepts: the alpha lengths of scales 0
+    and 1 (2B - 1) need Bluestein at M = 4096 on the generic FFT kernels, and
+    at B = 1000 so does the beta length 2B (oracle: ~15 s on 16 cores)."""
+    N, alpha = 2, 2.0
+    say(f"parity B={B} N={N} full J")
+    st = setup(B, N, alpha)
+    f = seeded_flm(B, 4321 + B, red=True)
+    w_ref, sc_ref = st.analysis(f)
+    w, sc = core.s2dw_core_analysis_flm2wav_dynamic(f, st.K, st.S, st.J, B, N, st.bl_scoeff, alpha)
+    errs = [float(np.abs(a - b).max()) for a, b in zip(w, w_ref)]
+    say("  wav max abs err per scale", ["%.2e" % e for e in errs])
+    assert max(errs) <= TOL
+    np.testing.assert_array_equal(sc, sc_ref)
+    g = core.s2dw_core_synthesis_wav2flm_dynamic(w_ref, sc_ref, st.K, st.Phi2, st.S, st.J, B, N, st.bl_scoeff, alpha)
+    e_syn = float(np.abs(g - st.synthesis(w_ref, sc_ref)).max())
+    e_rt = float(np.abs(core.s2dw_core_synthesis_wav2flm_dynamic(w, sc, st.K, st.Phi2, st.S, st.J, B, N,
+                                                                  st.bl_scoeff, alpha) - f).max())
+    say(f"  synthesis vs oracle {e_syn:.2e}, round trip {e_rt:.2e}")
+    assert e_syn <= TOL and e_rt <= TOL
+
+
+def test_band_limit_above_1024_refused():
+    """B = 1025 needs Bluestein at M = 8192, whose two LDS buffers (256 KiB)
+    exceed the 160 KiB of LDS per workgroup: the plan is refused with
+    SIZE_INVALID before any transform runs."""
+    B, N, alpha = 1025, 2, 2.0
+    say(f"refusal B={B}")
+    J = core.s2dw_core_comp_Jmax(B, alpha)
+    K, Phi2, bl = core.s2dw_core_init_kernels(J, B, alpha)
+    S = core.s2dw_core_init_directionality(B, N)
+    f = seeded_flm(B, 5)
+    with pytest.raises(core.S2dwError) as e:
+        core.s2dw_core_analysis_flm2wav_dynamic(f, K, S, J, B, N, bl, alpha)
+    assert e.value.code == core.S2DW_ERROR_SIZE_INVALID
+    assert "LDS" in str(e.value)
+
+
 def test_invalid_bl_scoeff_raises():
     st = setup(32, 3, 2.0)
     f = seeded_flm(32, 1)
