@@ -62,6 +62,34 @@ extern "C" {
 #define S2DW_PREC_FP64 0  /* plain form: everything in FP64 */
 #define S2DW_PREC_SPLIT 1 /* split-precision contraction on MFMA (see plan docs) */
 
+/* Rounding of the FP64 plain form (OR-ed into the precision argument of
+ * s2dw_hip_plan_create).  The reference's own arithmetic rounds a few
+ * quantities coarsely: its Wigner planes (row factors from running sums of
+ * logarithms, s2dw_dl_mod.f90:139-146, 216-228), the phase e^{i(n-m)pi/2}
+ * (core_mod.f90:666, exp of a rounded product) and the basis of its direct
+ * beta sum (cos/sin of the rounded product m' beta_b, core_mod.f90:720-731).
+ * At B = 256 with white N(0,1) input each of these moves the wavelet
+ * coefficients by 1e-12 to 6e-12 (values up to ~70); with a red spectrum the
+ * moves are ~1e-14.
+ * S2DW_WIGNER_STREAMED (default, 0): Wigner values from exact-start
+ * recursions in registers, no plane stored; exact phases; beta sums as FFTs.
+ * The fastest form.
+ * S2DW_WIGNER_TABLE (16): Wigner planes with the reference's rounding --
+ * the values s2dw_dl_beta_operator produces, bit for bit -- generated on the
+ * device once per plan (8 * B(B+1)(2B+1)/6 bytes, 359 MB at B = 512) and read
+ * by the contraction kernels.
+ * S2DW_ROUNDING_REFERENCE (48): the table planes plus the reference's rounded
+ * phases and beta-sum basis (the beta sums as FP64 GEMMs with its cos/sin
+ * values instead of FFTs); the wavelet coefficients land within the FP64
+ * rounding of the sums of the reference's output.  The host-buffer entry
+ * points (the Fortran drop-in) use this form.  Runs eagerly (no graph replay).
+ * The table and reference forms need N <= 8 (at most 8 analysis columns) and
+ * the MFMA contraction kernels; other plans run the streamed form
+ * (s2dw_hip_plan_wigner_form tells which). */
+#define S2DW_WIGNER_STREAMED 0
+#define S2DW_WIGNER_TABLE 16
+#define S2DW_ROUNDING_REFERENCE 48
+
 /* Text of the last error raised on the calling thread ("" if none). */
 const char *s2dw_hip_last_error(void);
 
@@ -302,6 +330,15 @@ int s2dw_hip_server_stats(s2dw_hip_server_t server, long long *in_use_bytes, int
  * out2 (may be NULL) for the synthesis (m'-recursion) generator.
  * ------------------------------------------------------------------- */
 int s2dw_hip_wigner_plane(int l, double *out, double *out2);
+
+/* S2DW_WIGNER_STREAMED, S2DW_WIGNER_TABLE or S2DW_ROUNDING_REFERENCE: the
+ * form the plan runs. */
+int s2dw_hip_plan_wigner_form(s2dw_hip_plan_t plan);
+
+/* Diagnostic: plane l of a table-form plan, out[m' * (l+1) + m] =
+ * Delta^l_{m'm}(pi/2) for 0 <= m', m <= l (synchronous).  Returns
+ * S2DW_ERROR_ARG_INVALID for a streamed plan. */
+int s2dw_hip_plan_wigner_table(s2dw_hip_plan_t plan, int l, double *out);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libs2dw_hip.so")
-SOURCES = ["s2dw_setup.cpp", "s2dw_kernels.hip", "s2dw_capi.cpp", "s2dw_serve.cpp", "s2dw_stats.hip"]
+SOURCES = ["s2dw_setup.cpp", "s2dw_kernels.hip", "s2dw_capi.cpp", "s2dw_serve.cpp", "s2dw_stats.hip", "s2dw_planes.hip",
+           "s2dw_refround.hip"]
 ARCH = os.environ.get("S2DW_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -45,7 +46,8 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fopenmp", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fopenmp", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocblas",
+           "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

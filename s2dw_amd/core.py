@@ -48,6 +48,13 @@ ERROR_COMMENT = [
 
 PREC_FP64 = 0
 PREC_SPLIT = 1
+# rounding of the FP64 plain form (OR-ed into precision): streamed recursions
+# (default), Wigner planes with the reference's rounding, or those planes plus
+# the reference's rounded phases and beta-sum basis (the host-buffer entry
+# points use the last)
+WIGNER_STREAMED = 0
+WIGNER_TABLE = 16
+ROUNDING_REFERENCE = 48
 
 
 class S2dwError(RuntimeError):
@@ -121,6 +128,8 @@ def lib() -> C.CDLL:
         L.s2dw_hip_stat_moments_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.s2dw_hip_analysis_moments_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.s2dw_hip_plan_moments_fusable.argtypes = [C.c_void_p]
+        L.s2dw_hip_plan_wigner_form.argtypes = [C.c_void_p]
+        L.s2dw_hip_plan_wigner_table.argtypes = [C.c_void_p, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -337,6 +346,17 @@ class Plan:
     @property
     def device_bytes(self) -> int:
         return int(lib().s2dw_hip_plan_device_bytes(self.h))
+
+    @property
+    def wigner_form(self) -> int:
+        """WIGNER_TABLE or WIGNER_STREAMED: where the contractions take Delta from."""
+        return int(lib().s2dw_hip_plan_wigner_form(self.h))
+
+    def wigner_table(self, l: int) -> np.ndarray:
+        """Plane l of the plan's reference-rounding table, indexed [m', m]."""
+        out = np.zeros((l + 1, l + 1), dtype=np.float64)  # row-major: out[m', m]
+        _check(lib().s2dw_hip_plan_wigner_table(self.h, int(l), _p(out)), "s2dw_hip_plan_wigner_table")
+        return out
 
     def empty_wav(self, device="cuda"):
         import torch

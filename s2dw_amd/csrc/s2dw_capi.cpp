@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <memory>
@@ -59,9 +60,24 @@ std::mutex g_mu;
 std::list<std::unique_ptr<HostEntry>> g_cache;  // most recent first
 constexpr size_t kCacheMax = 4;
 
+// Rounding form of the host-buffer entry points: the reference's own
+// (default), or S2DW_HIP_ROUNDING=streamed / table for the other forms of
+// the plain transform (read at each call).
+int host_form() {
+  const char *ev = std::getenv("S2DW_HIP_ROUNDING");
+  if (ev && !std::strcmp(ev, "streamed")) return S2DW_WIGNER_STREAMED;
+  if (ev && !std::strcmp(ev, "table")) return S2DW_WIGNER_TABLE;
+  return S2DW_ROUNDING_REFERENCE;
+}
+
+int plan_form(const Plan &p) {
+  return p.refround ? S2DW_ROUNDING_REFERENCE : p.d_Sa ? S2DW_WIGNER_TABLE : S2DW_WIGNER_STREAMED;
+}
+
 bool same_config(const Plan &p, int dev, int B, int N, int J, double alpha, const double *K, const double *Phi2,
                  const double *S) {
   if (p.device != dev || p.B != B || p.N != N || p.J != J || p.alpha != alpha) return false;
+  if (p.form_req != host_form()) return false;
   if (std::memcmp(p.K.data(), K, sizeof(double) * (size_t)(J + 1) * B)) return false;
   if (Phi2 && std::memcmp(p.Phi2.data(), Phi2, sizeof(double) * (size_t)B)) return false;
   if (std::memcmp(p.S.data(), S, sizeof(double) * 2 * (size_t)B * N)) return false;
@@ -84,7 +100,10 @@ int get_entry(int B, int N, int J, double alpha, const double *K, const double *
   auto e = std::make_unique<HostEntry>();
   std::vector<double> phi(B, 0.0);
   if (Phi2) std::memcpy(phi.data(), Phi2, sizeof(double) * B);
-  int err = plan_build(&e->plan, B, N, J, alpha, K, phi.data(), S, S2DW_PREC_FP64, nullptr, true);
+  // the drop-in reproduces the reference's own rounding unless asked otherwise
+  const int form = host_form();
+  int err = plan_build(&e->plan, B, N, J, alpha, K, phi.data(), S, S2DW_PREC_FP64 | form, nullptr, true);
+  e->plan.form_req = form;
   if (err) return err;
   const Plan &p = e->plan;
   S2DW_HIP_CHECK(hipMalloc(&e->d_flm, sizeof(double2) * (size_t)B * B));
@@ -264,8 +283,9 @@ int s2dw_hip_synthesis_wav2flm_dynamic(double *flm, const double *const *wavdyn,
 int s2dw_hip_plan_create(s2dw_hip_plan_t *plan, int B, int N, int J, double alpha, const double *K_gamma,
                          const double *Phi2, const double *Slm, int precision, void *stream) {
   if (!plan || !K_gamma || !Phi2 || !Slm) return kErrArgInvalid;
-  if (precision != S2DW_PREC_FP64 && precision != S2DW_PREC_SPLIT) return kErrArgInvalid;
-  if (precision == S2DW_PREC_SPLIT) {
+  const int prec = precision & ~S2DW_ROUNDING_REFERENCE;
+  if (prec != S2DW_PREC_FP64 && prec != S2DW_PREC_SPLIT) return kErrArgInvalid;
+  if (prec == S2DW_PREC_SPLIT) {
     set_error("the split-precision contraction is not available in this build; use S2DW_PREC_FP64");
     return kErrArgInvalid;
   }
@@ -324,6 +344,24 @@ int s2dw_hip_plan_sync(s2dw_hip_plan_t plan) {
 void *s2dw_hip_plan_stream(s2dw_hip_plan_t plan) { return plan ? (void *)plan->p.stream : nullptr; }
 
 int s2dw_hip_wigner_plane(int l, double *out, double *out2) { return wigner_plane(l, out, out2); }
+
+int s2dw_hip_plan_wigner_form(s2dw_hip_plan_t plan) {
+  if (!plan) return -1;
+  return plan_form(plan->p);
+}
+
+int s2dw_hip_plan_wigner_table(s2dw_hip_plan_t plan, int l, double *out) {
+  if (!plan || !out) return kErrArgInvalid;
+  const Plan &p = plan->p;
+  if (!p.d_Sa || l < 0 || l >= p.B) {
+    set_error("the plan has no Wigner table, or l=%d is outside [0, %d)", l, p.B);
+    return kErrArgInvalid;
+  }
+  const long long off = (long long)l * (l + 1) * (2LL * l + 1) / 6;
+  S2DW_HIP_CHECK(hipStreamSynchronize(p.stream));
+  S2DW_HIP_CHECK(hipMemcpy(out, p.d_Sa + off, sizeof(double) * (size_t)(l + 1) * (l + 1), hipMemcpyDeviceToHost));
+  return kErrNone;
+}
 
 int s2dw_hip_plan_set_timing(s2dw_hip_plan_t plan, int enable) {
   if (!plan) return kErrArgInvalid;

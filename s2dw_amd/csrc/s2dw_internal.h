@@ -106,6 +106,23 @@ struct Plan {
   // device tables
   double *d_S0 = nullptr;     // chain start values, triangle [l0][k]
   double *d_thin = nullptr;   // Delta^l_{m'k}, [tri(m',l)][k]
+  double *d_thinneg = nullptr;  // Delta^l_{m',-k}, [tri(m',l)][k] (table form only)
+  // table form of the plain transform: Wigner planes with the reference's
+  // rounding, [off(l) + m'(l+1) + m] (s2dw_planes.hip), read by the MFMA
+  // contraction kernels instead of the device recursions
+  double *d_Sa = nullptr;
+  long long sa_doubles = 0;
+  // reference-rounding form (table planes plus the reference's own phase
+  // factors and beta-sum basis values, s2dw_refround.hip)
+  bool refround = false;
+  int form_req = -1;           // form asked for by the host-buffer entry points' cache
+  double2 *d_Gref = nullptr;   // analysis gamma matrix as the reference rounds it, [g][c]
+  double2 *d_phref = nullptr;  // e^{i k pi/2} as the reference rounds it, k = n - m + phoff
+  int phoff = 0;
+  double *d_eta = nullptr;     // per scale: Ec[b][m'], Es[b][m'] (2hi x hi each)
+  std::vector<long long> eta_off;
+  double2 *d_cs = nullptr;     // sine-GEMM result of one scale
+  void *blas = nullptr;        // rocblas_handle
   double *d_Ba = nullptr;     // analysis rows [tri(m',l)][2 + 2*NCa]
   long long ba_bytes = 0;     // its size in bytes
   double *d_Bs = nullptr;     // synthesis rows [tri(l,m')][2 + 2*NCs]
@@ -211,5 +228,13 @@ int moments_run(hipStream_t st, const long long *off, const long long *count, in
 int moments_finish(hipStream_t st, const long long *count, const long long *rec0, int nscales, const double *d_part,
                    double *d_out);
 int wigner_plane(int l, double *out, double *out2);
+// Reference-rounding Wigner planes (s2dw_planes.hip): doubles of the table for
+// band limit L, and its construction into d_Sa with the exact thin columns.
+long long ref_plane_doubles(int L);
+int build_ref_planes(int L, int nk, double *d_Sa, double *d_thin, double *d_thinneg, hipStream_t st);
+// Reference-rounding beta sum of the analysis (s2dw_refround.hip).
+int refbeta_build(Plan *p);
+void refbeta_free(Plan *p);
+int refbeta_run(Plan *p, hipStream_t st);
 
 }  // namespace s2dw

@@ -1,7 +1,9 @@
 !------------------------------------------------------------------------------
 ! s2dw_hip_test -- round-trip check of the Fortran API on the GPU library.
 !
-! Usage: s2dw_hip_test B [N] [alpha] [J_min]
+! Usage: s2dw_hip_test B [N] [alpha] [J_min] [tol]
+!
+! tol (default 1e-12) is the reconstruction error the run must stay below.
 !
 ! Follows the steps of a typical S2DW host program: kernels, directionality,
 ! admissibility, dynamic analysis, dynamic synthesis, then reports the
@@ -19,14 +21,14 @@ program s2dw_hip_test
 
   character(len=64) :: arg
   integer :: B, N, J, J_min, bl_scoeff, el, m, nargs
-  real(dp) :: alpha, err
+  real(dp) :: alpha, err, tol
   integer(8) :: state, t0, t1, t2, rate
   complex(dpc), allocatable :: flm(:,:), flm_syn(:,:), Slm(:,:), scoeff(:,:)
   real(dp), allocatable :: K_gamma(:,:), Phi2(:), admiss(:)
   type(s2dw_wav_abg), allocatable :: wavdyn(:)
 
   nargs = command_argument_count()
-  B = 64; N = 3; alpha = 2d0; J_min = 0
+  B = 64; N = 3; alpha = 2d0; J_min = 0; tol = 1d-12
   if (nargs >= 1) then
      call get_command_argument(1, arg); read(arg,*) B
   end if
@@ -38,6 +40,9 @@ program s2dw_hip_test
   end if
   if (nargs >= 4) then
      call get_command_argument(4, arg); read(arg,*) J_min
+  end if
+  if (nargs >= 5) then
+     call get_command_argument(5, arg); read(arg,*) tol
   end if
   J = s2dw_core_comp_Jmax(B, alpha) - J_min
 
@@ -73,7 +78,7 @@ program s2dw_hip_test
      write(*,'(a)') 'wavdyn not freed by synthesis'
      stop 2
   end if
-  if (err < 1d-12) then
+  if (err < tol) then
      write(*,'(a)') 'Tests passed!'
   else
      write(*,'(a)') 'Tests failed!'

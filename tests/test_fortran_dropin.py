@@ -70,16 +70,27 @@ def run(cmd):
     return subprocess.run(cmd, capture_output=True, text=True, timeout=600)
 
 
+# The harness's signal is white (uniform in (-1,1)).  The drop-in's default
+# form reproduces the reference's own rounding, and with it the reference's
+# reconstruction error, which on white input grows past 1e-12 at large B (the
+# oracle: 2.1e-12 at B = 256, N = 5; this harness: 2.1e-12 at B = 512, N = 8):
+# those runs are held to 3e-12.  The streamed form (S2DW_HIP_ROUNDING=streamed)
+# reconstructs to 1e-12 everywhere.
 @pytest.mark.gpu
-@pytest.mark.parametrize("args", [["64"], ["64", "5", "3.0"], ["256", "5"], ["512", "8", "2.0", "2"]])
-def test_fortran_round_trip(args):
+@pytest.mark.parametrize("args,form,tol", [
+    (["64", "3", "2.0", "0"], "reference", 1e-12), (["64", "5", "3.0", "0"], "reference", 1e-12),
+    (["256", "5", "2.0", "0"], "reference", 3e-12), (["512", "8", "2.0", "2"], "reference", 3e-12),
+    (["256", "5", "2.0", "0"], "streamed", 1e-12), (["512", "8", "2.0", "2"], "streamed", 1e-12)])
+def test_fortran_round_trip(args, form, tol):
     assert os.path.exists(HIP_TEST), "build with `make -C s2dw_amd/fortran`"
-    r = run([HIP_TEST, *args])
+    cmd = [HIP_TEST, *args, str(tol)]
+    print(" ".join(cmd), "S2DW_HIP_ROUNDING=" + form, flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600, env=dict(os.environ, S2DW_HIP_ROUNDING=form))
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Tests passed!" in r.stdout
     err = float(re.search(r"max reconstruction error =\s*([0-9.Ee+-]+)", r.stdout).group(1))
-    assert err < 1e-12
+    assert err < tol
 
 
 @pytest.mark.gpu

@@ -1,7 +1,7 @@
 """The request loop (s2dw_hip_server_*): analyses and syntheses of mixed
 shapes submitted together, some synthesising from coefficients an earlier
-analysis left resident, give the same results as the single-call entry
-points, bit for bit (tolerance 0: every request runs the same kernels on
+analysis left resident, give the same results as each request alone on a
+device plan of the same form, bit for bit (tolerance 0: every request runs the same kernels on
 buffers of its own).  A small HBM budget forces queueing and plan eviction
 without changing any result."""
 import sys
@@ -34,9 +34,21 @@ def setup(B, N, alpha, J):
 
 
 def alone(B, N, alpha, J, flm):
+    """The same request alone: one device plan of the server's form (the
+    default, streamed), analysis then synthesis of its output."""
+    import torch
+
     J, K, Phi2, S, bl = setup(B, N, alpha, J)
-    w, sc = core.s2dw_core_analysis_flm2wav_dynamic(flm, K, S, J, B, N, bl, alpha)
-    g = core.s2dw_core_synthesis_wav2flm_dynamic(w, sc, K, Phi2, S, J, B, N, bl, alpha)
+    plan = core.Plan(B, N, J, alpha, K, Phi2, S)
+    d_f = torch.from_numpy(np.ascontiguousarray(flm.T)).cuda()
+    d_w, d_sc, d_g = plan.empty_wav(), plan.empty_scoeff(), plan.empty_flm()
+    plan.analysis(d_f, d_w, d_sc)
+    plan.synthesis(d_w, d_sc, d_g)
+    plan.sync()
+    w = [np.asfortranarray(np.transpose(plan.scale_view(d_w, j).cpu().numpy(), (2, 1, 0))) for j in range(J + 1)]
+    sc = np.asfortranarray(d_sc.cpu().numpy().T)
+    g = np.asfortranarray(d_g.cpu().numpy().T)
+    plan.close()
     return J, w, sc, g
 
 
