@@ -64,6 +64,8 @@ def lib():
                                        _dp, C.POINTER(_dp), _dp, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_dl_precise.argtypes = [C.c_int]
+        L.orc_set_exact_phase.argtypes = [C.c_int]
+        L.orc_set_exact_beta.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -79,6 +81,16 @@ def set_threads(n: int) -> None:
 def set_dl_precise(on: bool) -> None:
     """Diagnostic: compute Wigner planes with the same recursion in long double."""
     lib().orc_set_dl_precise(int(bool(on)))
+
+
+def set_exact_phase(on: bool) -> None:
+    """Diagnostic: exact powers of i instead of the reference's rounded exp(I*k*PION2)."""
+    lib().orc_set_exact_phase(int(bool(on)))
+
+
+def set_exact_beta(on: bool) -> None:
+    """Diagnostic: exact beta-sum basis instead of cos/sin of the rounded m'*beta_b."""
+    lib().orc_set_exact_beta(int(bool(on)))
 
 
 def get_threads() -> int:

@@ -417,6 +417,21 @@ void orc_dl_beta_operator(double *dl, double beta, int l) {
   orc_dl_fill(dl, l);
 }
 
+/* Diagnostic switches (default 0, the reference's arithmetic): 1 replaces
+ * the reference's phase factor exp(I*(n-m)*PION2) (core_mod.f90:666, 975)
+ * by the exact power of i, resp. its beta-sum basis 2cos(m' beta_b),
+ * 2 sin(m' beta_b) (core_mod.f90:720-731, evaluated at the rounded product
+ * m' * beta_b) by the exact values (argument reduced in integers, long
+ * double).  They measure how far each of the reference's roundings moves its
+ * output. */
+static int orc_exact_phase = 0, orc_exact_beta = 0;
+void orc_set_exact_phase(int on) { orc_exact_phase = on; }
+void orc_set_exact_beta(int on) { orc_exact_beta = on; }
+static cplx orc_ipow(int k) {
+  const int r = ((k % 4) + 4) % 4;
+  return r == 0 ? 1.0 : r == 1 ? I : r == 2 ? -1.0 : -I;
+}
+
 /* ------------------------------------------------------------------------
  * FFTW substitute (FFTW's documented transform definitions).
  * ---------------------------------------------------------------------- */
@@ -576,7 +591,8 @@ static int orc_analysis_scale(double *wj, const cplx *flm, const double *K_gamma
                 sb = conj(KG(jj, el) * SLM(el, mmm)) * FLM(el, m);
               }
               T[m + (size_t)hi * (mm + (size_t)hi * (mmm + N - 1))] +=
-                  cexp(I * (double)(mmm - m) * ORC_PION2) * DL(mm, m) * DL(mm, mmm) * sb;
+                  (orc_exact_phase ? orc_ipow(mmm - m) : cexp(I * (double)(mmm - m) * ORC_PION2)) * DL(mm, m) *
+                  DL(mm, mmm) * sb;
             }
           }
         }
@@ -624,9 +640,13 @@ static int orc_analysis_scale(double *wj, const cplx *flm, const double *K_gamma
       for (int mm = 0; mm <= hi - 1; mm++) {
         for (int m = 0; m <= hi - 1; m++) {
           cplx eta;
-          if (mm == 0)
+          if (mm == 0) {
             eta = 1.0;
-          else if ((m + N - 1) % 2 == 0)
+          } else if (orc_exact_beta) {
+            const long double ang = 3.141592653589793238462643383279502884L *
+                                    (long double)((long)mm * (2 * bb + 1) % (8L * hi)) / (4.0L * hi);
+            eta = ((m + N - 1) % 2 == 0) ? (double)(2.0L * cosl(ang)) : I * (double)(2.0L * sinl(ang));
+          } else if ((m + N - 1) % 2 == 0)
             eta = 2.0 * cos(mm * beta_b);
           else
             eta = I * 2.0 * sin(mm * beta_b);
@@ -802,7 +822,7 @@ static int orc_synthesis_scale(cplx *flm, const double *wj, const double *K_gamm
           if (k_indicator > 0) {
             cplx s = 0;
             for (int mm = 0; mm <= el; mm++) s += VV(m, mm + hi - 1, mmm) * DL(mm, m) * DL(mm, mmm);
-            WIG(el, m, mmm) = s * cexp(-I * (double)(mmm - m) * ORC_PION2);
+            WIG(el, m, mmm) = s * (orc_exact_phase ? conj(orc_ipow(mmm - m)) : cexp(-I * (double)(mmm - m) * ORC_PION2));
           }
         }
       }

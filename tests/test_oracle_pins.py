@@ -283,3 +283,42 @@ def test_ran2_test_signal_is_deterministic():
     np.testing.assert_array_equal(f1, f2)
     assert np.all(np.abs(f1.real) <= 1) and np.all(np.triu(f1, 1) == 0)
     assert np.all(f1.imag[:, 0] == 0)
+
+
+def test_reference_rounding_budget_white_L256():
+    """How far each of the reference's own roundings moves its output on the
+    white C2 input (L=256, N=5, alpha=2, full J): the Wigner planes (row
+    factors from running sums of logarithms), the phase exp(I*(n-m)*PION2) and
+    the beta-sum basis cos/sin(m' beta_b) at the rounded product.  Each is
+    switched to exact arithmetic in turn; the moves are several times the
+    1e-12 tolerance on scales 0-1 (values up to ~70), which is why the GPU's
+    drop-in form reproduces all three rather than computing more exactly.
+    Measured: planes 5.6e-12, beta basis 2.8e-12, phase 1.0e-12 on scale 0;
+    the reconstruction error of the reference itself on this input is 2.1e-12."""
+    from conftest import seeded_flm
+
+    B, N = 256, 5
+    st = O.Setup(B, N, 2.0)
+    f = seeded_flm(B, 2, red=False)
+    w0, sc0 = st.analysis(f)
+    rec = float(np.abs(st.synthesis(w0, sc0) - f).max())
+
+    def move(**kw):
+        sw = {"planes": O.set_dl_precise, "phase": O.set_exact_phase, "beta": O.set_exact_beta}
+        for k in kw:
+            sw[k](True)
+        try:
+            w1, _ = st.analysis(f)
+        finally:
+            for k in kw:
+                sw[k](False)
+        return [float(np.abs(a - b).max()) for a, b in zip(w0, w1)]
+
+    planes, phase, beta = move(planes=1), move(phase=1), move(beta=1)
+    print("planes", planes[:3], "phase", phase[:3], "beta", beta[:3], "reconstruction", rec)
+    assert 4e-12 < planes[0] < 8e-12
+    assert 1.5e-12 < beta[0] < 4e-12
+    assert 5e-13 < phase[0] < 2e-12
+    assert 1.5e-12 < rec < 3e-12
+    # the small scales carry little: every move is below 1e-12 from scale 3 on
+    assert max(planes[3:] + phase[3:] + beta[3:]) < 1e-12
