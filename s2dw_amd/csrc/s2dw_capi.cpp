@@ -285,10 +285,6 @@ int s2dw_hip_plan_create(s2dw_hip_plan_t *plan, int B, int N, int J, double alph
   if (!plan || !K_gamma || !Phi2 || !Slm) return kErrArgInvalid;
   const int prec = precision & ~S2DW_ROUNDING_REFERENCE;
   if (prec != S2DW_PREC_FP64 && prec != S2DW_PREC_SPLIT) return kErrArgInvalid;
-  if (prec == S2DW_PREC_SPLIT) {
-    set_error("the split-precision contraction is not available in this build; use S2DW_PREC_FP64");
-    return kErrArgInvalid;
-  }
   auto *h = new (std::nothrow) s2dw_hip_plan;
   if (!h) return kErrMemAlloc;
   const int e = plan_build(&h->p, B, N, J, alpha, K_gamma, Phi2, Slm, precision, (hipStream_t)stream, false);

@@ -112,10 +112,16 @@ struct Plan {
   // contraction kernels instead of the device recursions
   double *d_Sa = nullptr;
   long long sa_doubles = 0;
+  // split-precision analysis contraction (S2DW_PREC_SPLIT): digits of the
+  // analysis columns [rowA(m',l)][16] and the per-transform row exponents
+  uint4 *d_Bq = nullptr;
+  long long bq_bytes = 0;
+  int *d_erow = nullptr;
   // reference-rounding form (table planes plus the reference's own phase
   // factors and beta-sum basis values, s2dw_refround.hip)
   bool refround = false;
   int form_req = -1;           // form asked for by the host-buffer entry points' cache
+  bool tab_syn = false;        // the synthesis contraction reads the Wigner table too
   double2 *d_Gref = nullptr;   // analysis gamma matrix as the reference rounds it, [g][c]
   double2 *d_phref = nullptr;  // e^{i k pi/2} as the reference rounds it, k = n - m + phoff
   int phoff = 0;
