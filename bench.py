@@ -87,25 +87,60 @@ def stage_bytes(hi, lo, N, ncs, grouped=True):
     return b
 
 
-def cpu_baseline(K, Phi2, S, J, bl, flm):
-    """The CPU oracle (a C/OpenMP restatement of the reference's algorithm) on the
-    host's cores: one analysis + one synthesis of the same workload."""
+def host_cpu():
+    """nproc, the cores this process may run on, sockets and the CPU model."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        with open("/proc/cpuinfo") as fh:
+            txt = fh.read()
+        models = [ln.split(":", 1)[1].strip() for ln in txt.splitlines() if ln.startswith("model name")]
+        phys = {ln.split(":", 1)[1].strip() for ln in txt.splitlines() if ln.startswith("physical id")}
+        info["model"] = models[0] if models else "unknown"
+        info["sockets"] = max(1, len(phys))
+    except OSError:
+        info["model"], info["sockets"] = "unknown", None
+    return info
+
+
+def cpu_baseline(K, Phi2, S, J, bl, flm, rounds=3):
+    """The CPU oracle (a C/OpenMP restatement of the reference's algorithm,
+    loop for loop) on the host's cores, as the reference's timing is taken
+    (s2dw_test.f90:114-135: wall clock around each call, kernels built
+    before): one warm-up round trip, then the median of `rounds` analyses and
+    of `rounds` syntheses.  Threads: every core this process may use, capped
+    by OMP_NUM_THREADS when that is set (the job's CPU share).  Returns the
+    record and the oracle's coefficients (the FP64 reference for the
+    per-scale errors)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    cores = min(16, os.cpu_count() or 1)
+    hw = host_cpu()
+    cores = hw["affinity"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
     O.set_threads(cores)
-    t0 = time.perf_counter()
-    w, sc = O.analysis_dynamic(flm, K, S, J, L, N, bl, ALPHA)
-    t1 = time.perf_counter()
-    g = O.synthesis_dynamic(w, sc, K, Phi2, S, J, L, N, bl, ALPHA)
-    t2 = time.perf_counter()
+    w, sc = O.analysis_dynamic(flm, K, S, J, L, N, bl, ALPHA)  # warm-up
+    O.synthesis_dynamic(w, sc, K, Phi2, S, J, L, N, bl, ALPHA)
+    ta, ts = [], []
+    for _ in range(rounds):
+        t0 = time.perf_counter()
+        w, sc = O.analysis_dynamic(flm, K, S, J, L, N, bl, ALPHA)
+        t1 = time.perf_counter()
+        g = O.synthesis_dynamic(w, sc, K, Phi2, S, J, L, N, bl, ALPHA)
+        t2 = time.perf_counter()
+        ta.append(t1 - t0)
+        ts.append(t2 - t1)
+        print(f"bench: cpu baseline round {len(ta)}: analysis {ta[-1]:.2f} s, synthesis {ts[-1]:.2f} s",
+              file=sys.stderr, flush=True)
     err = float(np.abs(g - flm).max())
-    return {"value": 2.0 / (t2 - t0), "unit": "transforms/s", "cores": cores, "kind": "port",
-            "sample": f"one analysis ({t1 - t0:.1f} s) + one synthesis ({t2 - t1:.1f} s) at L=512 N=8 J=7 "
-                      f"by oracle/s2dw_oracle.c (reference loop structure, OpenMP, {cores} threads); "
-                      f"round-trip err {err:.1e}",
-            "analysis_s": t1 - t0, "synthesis_s": t2 - t1}
+    a_med, s_med = statistics.median(ta), statistics.median(ts)
+    rec = {"value": 2.0 / (a_med + s_med), "unit": "transforms/s", "cores": cores, "kind": "port",
+           "sample": f"L=512 N=8 J=7 round trips by oracle/s2dw_oracle.c (the reference's loop structure, "
+                     f"OpenMP, {cores} threads): one warm-up, then the median of {rounds} analyses "
+                     f"({a_med:.2f} s) and of {rounds} syntheses ({s_med:.2f} s); round-trip err {err:.1e}",
+           "analysis_s": a_med, "synthesis_s": s_med, "analysis_runs_s": ta, "synthesis_runs_s": ts,
+           "host": hw, "planning_in_timed_region": "no FFT planning (the oracle's FFT has none)"}
+    return rec, w
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
@@ -190,17 +225,18 @@ def _mix_requests(rng, count, Ls, Ns, alphas, log_uniform=False):
     return reqs
 
 
-def bench_batch(args):
+def bench_batch(args, precision=0):
     """64 transforms (32 analyses, 32 syntheses of their resident outputs),
     B from {64,128,256,512}, N from {1,3,5,8}, alpha = 2, J = J_max, all
     submitted at once to the request loop.  Two warm passes (the first builds
     the plans, the second still grows the pinned staging pool), then the
-    median of three timed passes."""
+    median of three timed passes.  Plan builds are timed separately (the
+    first warm pass)."""
     from s2dw_amd.serve import Server
 
     rng = np.random.default_rng(4)
     reqs = _mix_requests(rng, 32, [64, 128, 256, 512], [1, 3, 5, 8], [2.0])
-    with Server(200 << 30, 16) as srv:
+    with Server(200 << 30, 16, precision) as srv:
         def one_pass():
             t0 = time.perf_counter()
             ta = [srv.submit_analysis(B, N, J, a, f, keep_resident=True, want_wav=False) for (B, N, a, J, f) in reqs]
@@ -221,11 +257,12 @@ def bench_batch(args):
            "transforms_per_s": 64 / wall, "wall_ms": 1e3 * wall, "max_round_trip_err": err,
            "timed_passes_ms": [1e3 * r[0] for r in runs], "warm_passes_ms": [1e3 * w for w in warm],
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-           "dtype": "f64", "n_gpus": 1}
-    print(json.dumps(out), flush=True)
+           "first_pass_ms_with_plan_builds": 1e3 * warm[0],
+           "precision": "split" if precision & 0xF else "fp64", "dtype": "f64", "n_gpus": 1}
+    return out
 
 
-def bench_serve(args):
+def bench_serve(args, precision=0):
     """Request loop under Poisson arrivals: B log-uniform in [32, 1024]
     (rounded), N uniform in 1..8, alpha in {2, 3}, J = J_max; every request
     is an analysis whose coefficients stay resident, half of them followed by
@@ -240,7 +277,7 @@ def bench_serve(args):
     reqs = _mix_requests(rng, n_req, [32, 1024], [1, 8], [2.0, 3.0], log_uniform=True)
     follow = rng.random(n_req) < 0.5
     gaps = rng.exponential(1.0 / rate, n_req)
-    with Server(200 << 30, 16) as srv:
+    with Server(200 << 30, 16, precision) as srv:
         def one_pass(realtime):
             tickets, pairs, lock = [], [], threading.Lock()
             total = n_req + int(follow.sum())
@@ -280,7 +317,9 @@ def bench_serve(args):
                     srv.release(t)
             err = max(float(np.abs(ts.out - f).max()) for ts, f in pairs) if pairs else 0.0
             return t1 - t0, [t.latency_ms for t in tickets], tickets, err
-        one_pass(False)
+        tb0 = time.perf_counter()
+        one_pass(False)  # builds every plan of the stream
+        t_build = time.perf_counter() - tb0
         wall, lat, tickets, err = one_pass(True)
     ntr = len(tickets)
     out = {"workload": f"serving loop: {n_req} requests, Poisson arrivals at {rate}/s, B log-uniform in [32,1024], "
@@ -289,8 +328,9 @@ def bench_serve(args):
            "transforms": ntr, "transforms_per_s": ntr / wall, "wall_s": wall,
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                           "max": float(np.max(lat))},
-           "max_round_trip_err": err, "dtype": "f64", "n_gpus": 1}
-    print(json.dumps(out), flush=True)
+           "max_round_trip_err": err, "warm_pass_s_with_plan_builds": t_build,
+           "precision": "split" if precision & 0xF else "fp64", "dtype": "f64", "n_gpus": 1}
+    return out
 
 
 # stage name -> kernel whose launches dominate it
@@ -361,6 +401,83 @@ def bench_pipelined(args, L, N, J, K, Phi2, S, prec, d_f, flm):
             "note": "analysis of step i+1 overlapping synthesis of step i (two plans, two streams)"}
 
 
+def time_form(args, J, K, Phi2, S, prec, d_f, flm, steps):
+    """Round trips of one precision / rounding form of the same workload:
+    transforms/s over `steps` back-to-back round trips (graph replay unless the
+    form runs eagerly), stage times of its contractions, round-trip error,
+    and its wavelet coefficients (host copy, packed)."""
+    import torch
+
+    from s2dw_amd import core
+
+    stream = torch.cuda.current_stream()
+    t_b0 = time.perf_counter()
+    plan = core.Plan(L, N, J, ALPHA, K, Phi2, S, precision=prec, stream=stream)
+    build_ms = 1e3 * (time.perf_counter() - t_b0)
+    tuned = plan.autotune(2) if not args.no_autotune else "not run"
+    d_w, d_sc, d_g = plan.empty_wav(), plan.empty_scoeff(), plan.empty_flm()
+    plan.set_graphs(not args.eager)
+    for _ in range(max(1, args.warmup)):
+        plan.analysis(d_f, d_w, d_sc)
+        plan.synthesis(d_w, d_sc, d_g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.analysis(d_f, d_w, d_sc)
+        plan.synthesis(d_w, d_sc, d_g)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    plan.set_timing(True)
+    for _ in range(3):
+        plan.analysis(d_f, d_w, d_sc)
+        plan.synthesis(d_w, d_sc, d_g)
+    torch.cuda.synchronize()
+    plan.set_timing(False)
+    st = plan.stage_times()
+    rt = float(np.abs(d_g.cpu().numpy().T - flm).max())
+    wav = d_w.cpu().numpy()
+    rec = {"transforms_per_s": 2.0 * steps / wall, "ms_per_step": 1e3 * wall / steps,
+           "round_trip_max_err": rt, "plan_device_bytes": plan.device_bytes, "plan_build_ms": build_ms,
+           "wigner_form": {0: "streamed", 16: "table", 48: "reference"}.get(plan.wigner_form, "?"),
+           "contraction_ms": {k: st[k][0] / 3 for k in ("ana_contract", "syn_contract") if k in st},
+           "autotune": tuned}
+    offs, his = plan.wav_offsets, plan.bl_hi
+    plan.close()
+    return rec, wav, offs, his
+
+
+def per_scale_err(wav, ref_scales, offs, his):
+    """Max abs difference per scale between packed coefficients and per-scale
+    (a, b, g) arrays."""
+    out = []
+    for j, h in enumerate(his):
+        n = (2 * h - 1) * 2 * h * N
+        x = wav[offs[j]:offs[j] + n].reshape((N, 2 * h, 2 * h - 1))
+        out.append(float(np.abs(np.transpose(x, (2, 1, 0)) - ref_scales[j]).max()))
+    return out
+
+
+def host_api_ms(J, K, Phi2, S, bl, flm):
+    """The Fortran drop-in's cost: the host-buffer entry points at this
+    workload (reference-rounding form, PCIe copies of f_lm and of the 156 MB
+    of coefficients each way), median of three after one warm call."""
+    from s2dw_amd import core
+
+    ta, ts = [], []
+    for i in range(4):
+        t0 = time.perf_counter()
+        w, sc = core.s2dw_core_analysis_flm2wav_dynamic(flm, K, S, J, L, N, bl, ALPHA)
+        t1 = time.perf_counter()
+        core.s2dw_core_synthesis_wav2flm_dynamic(w, sc, K, Phi2, S, J, L, N, bl, ALPHA)
+        t2 = time.perf_counter()
+        if i:
+            ta.append(1e3 * (t1 - t0))
+            ts.append(1e3 * (t2 - t1))
+    return {"analysis_ms": statistics.median(ta), "synthesis_ms": statistics.median(ts),
+            "note": "host-buffer entry points (the Fortran drop-in): reference-rounding form, f_lm and the "
+                    "wavelet coefficients copied over PCIe each way; median of 3 after a warm call"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -381,6 +498,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel paths")
     ap.add_argument("--pipeline", action="store_true", help="also measure round trips with the analysis of step "
                                                             "i+1 overlapping the synthesis of step i")
+    ap.add_argument("--skip-forms", action="store_true", help="skip the split, table and reference-rounding forms")
+    ap.add_argument("--skip-serving", action="store_true", help="skip the batch (64 transforms) and serving "
+                                                                "(Poisson stream) workloads")
     args = ap.parse_args()
     if args.traffic_from:
         traffic_from(*args.traffic_from)
@@ -398,10 +518,10 @@ def main():
     from s2dw_amd import core
 
     if args.workload == "batch":
-        bench_batch(args)
+        print(json.dumps(bench_batch(args)), flush=True)
         return
     if args.workload == "serve":
-        bench_serve(args)
+        print(json.dumps(bench_serve(args)), flush=True)
         return
     dev = torch.device("cuda:0")
     J = core.s2dw_core_comp_Jmax(L, ALPHA) - J_MIN
@@ -497,6 +617,7 @@ def main():
                             ((d_mf[:6] - d_mo[:6]).abs() / d_mo[:6].abs().clamp(min=1.0)).max())})
 
     hi, lo = plan.bl_hi, plan.bl_lo
+    out_offs = list(plan.wav_offsets)
     ana_cmac, syn_cmac = contraction_cmacs(hi, lo, N)
     grouped = st.get("ana_small_ffts", (0.0, 0))[1] > 0
     sb = stage_bytes(hi, lo, N, ncs=sum(1 for n in range(N) if (N + n) % 2), grouped=grouped)
@@ -567,9 +688,42 @@ def main():
         "fft_stages": ffts,
         "moments": moments,
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(K, Phi2, S, J, bl, flm)
+    wav64 = d_w.cpu().numpy()
     plan.close()
+    print("bench: fp64 round trips done", file=sys.stderr, flush=True)
+    # the split-precision form (analysis contraction on int8 MFMA) and the
+    # table / reference-rounding forms of the plain transform, same workload
+    forms = {}
+    wavs = {}
+    for name, prec in (("split", core.PREC_SPLIT), ("table", core.WIGNER_TABLE),
+                       ("reference", core.ROUNDING_REFERENCE)):
+        if args.skip_forms:
+            break
+        forms[name], wavs[name], offs, his = time_form(args, J, K, Phi2, S, prec, d_f, flm, steps)
+        print(f"bench: form {name} done", file=sys.stderr, flush=True)
+    if "split" in forms:
+        sp = forms["split"]
+        sp["ratio_vs_fp64"] = sp["transforms_per_s"] / out["value"]
+        sp["max_err_per_scale_vs_gpu_fp64"] = per_scale_err(
+            wavs["split"], [np.transpose(wav64[o:o + (2 * h - 1) * 2 * h * N].reshape((N, 2 * h, 2 * h - 1)), (2, 1, 0))
+                            for o, h in zip(offs, his)], offs, his)
+    out["forms"] = forms
+    if not args.skip_forms:
+        out["host_api"] = host_api_ms(J, K, Phi2, S, bl, flm)
+        print("bench: host api done", file=sys.stderr, flush=True)
+    if not args.skip_serving:
+        out["batch"] = {"fp64": bench_batch(args, core.PREC_FP64), "split": bench_batch(args, core.PREC_SPLIT)}
+        print("bench: batch done", file=sys.stderr, flush=True)
+        out["serve"] = {"fp64": bench_serve(args, core.PREC_FP64), "split": bench_serve(args, core.PREC_SPLIT)}
+        print("bench: serve done", file=sys.stderr, flush=True)
+    if not args.no_cpu_baseline:
+        rec, w_orc = cpu_baseline(K, Phi2, S, J, bl, flm)
+        out["cpu_baseline"] = rec
+        # the errors per scale against the FP64 oracle (the reference's algorithm)
+        offs64 = [int(o) for o in out_offs]
+        out["max_err_per_scale_vs_oracle"] = {"fp64": per_scale_err(wav64, w_orc, offs64, hi)}
+        for name in wavs:
+            out["max_err_per_scale_vs_oracle"][name] = per_scale_err(wavs[name], w_orc, offs64, hi)
     print(json.dumps(out), flush=True)
 
 

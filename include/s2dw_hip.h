@@ -297,6 +297,12 @@ int s2dw_hip_plan_moments_fusable(s2dw_hip_plan_t plan);
 typedef struct s2dw_hip_server *s2dw_hip_server_t;
 
 int s2dw_hip_server_create(s2dw_hip_server_t *server, long long hbm_budget_bytes, int max_inflight);
+/* As s2dw_hip_server_create, with the precision (and rounding form) of every
+ * plan the server builds, as for s2dw_hip_plan_create: S2DW_PREC_FP64 (the
+ * default of s2dw_hip_server_create) or S2DW_PREC_SPLIT, OR-ed with a
+ * S2DW_WIGNER_* / S2DW_ROUNDING_REFERENCE form.  A request the precision
+ * cannot serve (S2DW_PREC_SPLIT with N > 8) fails with the plan's error. */
+int s2dw_hip_server_create_ex(s2dw_hip_server_t *server, long long hbm_budget_bytes, int max_inflight, int precision);
 /* Finishes queued and running requests, then frees everything. */
 int s2dw_hip_server_destroy(s2dw_hip_server_t server);
 

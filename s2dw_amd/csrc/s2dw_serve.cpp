@@ -128,6 +128,7 @@ struct s2dw_hip_server {
   int device = 0;
   long long budget = 0, in_use = 0;
   int max_inflight = 1;
+  int precision = S2DW_PREC_FP64;  // precision (and rounding form) of every plan the server builds
   long long next_ticket = 1;
   std::deque<std::shared_ptr<s2dw::Request>> queue;
   std::vector<std::shared_ptr<s2dw::Request>> inflight;
@@ -167,14 +168,14 @@ int bl_of(const Key &k) {
 }
 
 // Builds a plan for the key (host setup + device tables) on its own stream.
-int build_plan(const Key &k, PlanEntry **out) {
+int build_plan(const Key &k, int precision, PlanEntry **out) {
   auto *e = new (std::nothrow) PlanEntry;
   if (!e) return kErrMemAlloc;
   std::vector<double> K((size_t)(k.J + 1) * k.B), Phi2(k.B), S((size_t)2 * k.B * k.N);
   int bl = 0;
   int err = init_kernels(K.data(), Phi2.data(), &bl, k.J, k.B, k.alpha);
   if (!err) err = init_directionality(S.data(), k.B, k.N);
-  if (!err) err = plan_build(&e->p, k.B, k.N, k.J, k.alpha, K.data(), Phi2.data(), S.data(), S2DW_PREC_FP64, nullptr, true);
+  if (!err) err = plan_build(&e->p, k.B, k.N, k.J, k.alpha, K.data(), Phi2.data(), S.data(), precision, nullptr, true);
   if (!err) err = hchk(hipStreamSynchronize(e->p.stream), "plan set-up");
   if (err) {
     plan_free(&e->p);
@@ -398,7 +399,7 @@ bool admit_at(s2dw_hip_server *s, size_t qi, const void *src_wav, const void *sr
       if (s->inflight.empty()) return fail(kErrMemAlloc, "request buffers exceed the server's HBM budget");
       return false;
     }
-    const int err = build_plan(r->key, &plan);
+    const int err = build_plan(r->key, s->precision, &plan);
     if (err) return fail(err, s2dw_hip_last_error());
     s->in_use += plan->p.device_bytes;
     ++s->nplans[r->key];
@@ -586,9 +587,16 @@ using namespace s2dw;
 extern "C" {
 
 int s2dw_hip_server_create(s2dw_hip_server_t *srv, long long hbm_budget_bytes, int max_inflight) {
+  return s2dw_hip_server_create_ex(srv, hbm_budget_bytes, max_inflight, S2DW_PREC_FP64);
+}
+
+int s2dw_hip_server_create_ex(s2dw_hip_server_t *srv, long long hbm_budget_bytes, int max_inflight, int precision) {
   if (!srv || hbm_budget_bytes <= 0 || max_inflight < 1) return kErrArgInvalid;
+  const int prec = precision & ~S2DW_ROUNDING_REFERENCE;
+  if (prec != S2DW_PREC_FP64 && prec != S2DW_PREC_SPLIT) return kErrArgInvalid;
   auto *s = new (std::nothrow) s2dw_hip_server;
   if (!s) return kErrMemAlloc;
+  s->precision = precision;
   if (hipGetDevice(&s->device) != hipSuccess) {
     delete s;
     return kErrInitFail;

@@ -32,6 +32,7 @@ def _bind():
         return lib()
     L = lib()
     L.s2dw_hip_server_create.argtypes = [C.POINTER(C.c_void_p), _ll, C.c_int]
+    L.s2dw_hip_server_create_ex.argtypes = [C.POINTER(C.c_void_p), _ll, C.c_int, C.c_int]
     L.s2dw_hip_server_destroy.argtypes = [C.c_void_p]
     L.s2dw_hip_server_submit_analysis.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
                                                   C.c_int, C.POINTER(_ll)]
@@ -70,11 +71,13 @@ class Ticket:
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes: int, max_inflight: int = 8):
+    def __init__(self, hbm_budget_bytes: int, max_inflight: int = 8, precision: int = 0):
+        """precision: core.PREC_FP64 (default) or core.PREC_SPLIT, OR-ed with a
+        rounding form (core.WIGNER_TABLE, core.ROUNDING_REFERENCE)."""
         self._L = _bind()
         h = C.c_void_p()
-        _check(self._L.s2dw_hip_server_create(C.byref(h), int(hbm_budget_bytes), int(max_inflight)),
-               "s2dw_hip_server_create")
+        _check(self._L.s2dw_hip_server_create_ex(C.byref(h), int(hbm_budget_bytes), int(max_inflight),
+                                                 int(precision)), "s2dw_hip_server_create_ex")
         self._h = h
 
     def submit_analysis(self, B, N, J, alpha, flm, keep_resident=False, want_wav=True) -> Ticket:
