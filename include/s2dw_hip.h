@@ -272,6 +272,21 @@ int s2dw_hip_stat_moments_device(s2dw_hip_plan_t plan, double *d_wav, const doub
  * when enabled), no allocation.  Returns S2DW_ERROR_ARG_INVALID when
  * s2dw_hip_plan_moments_fusable is 0 for the plan. */
 int s2dw_hip_analysis_moments_device(s2dw_hip_plan_t plan, double *d_out, double *d_scoeff, const double *d_flm);
+/* No counterpart in the reference: s2dw_hip_stat_moments_device without a
+ * plan and without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  d_wav: the packed coefficients of (J, B, N, alpha) in
+ * the dynamic per-scale layout (scale jj at the offsets s2dw_hip_plan_sizes
+ * gives), masked in place when d_mask != NULL (packed alike); both 16-byte
+ * aligned.  d_out: 8*(J+1) doubles, rows as s2dw_hip_stat_moments_device.
+ * Enqueued on `stream` (a hipStream_t).  The per-chunk records live in a
+ * static device array of 1 MiB (131072 doubles) per GPU, held until the
+ * process ends: configurations needing more (e.g. B = 2048, N = 8) and J > 31
+ * return S2DW_ERROR_SIZE_INVALID.  Calls that overlap on one GPU share that
+ * array and race. */
+int s2dw_hip_stat_moments_device_noalloc(double *d_wav, const double *d_mask, int J, int B, int N, double alpha,
+                                         double *d_out, void *stream);
+
 /* 1 when every non-empty scale of the plan runs in the grouped or paired
  * FFT launches (all configurations with B <= 1024 on the workgroup engine),
  * which s2dw_hip_analysis_moments_device needs; 0 otherwise. */

@@ -129,6 +129,8 @@ def lib() -> C.CDLL:
         L.s2dw_hip_analysis_moments_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.s2dw_hip_plan_moments_fusable.argtypes = [C.c_void_p]
         L.s2dw_hip_plan_wigner_form.argtypes = [C.c_void_p]
+        L.s2dw_hip_stat_moments_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                                          C.c_double, C.c_void_p, C.c_void_p]
         L.s2dw_hip_plan_wigner_table.argtypes = [C.c_void_p, C.c_int, _dp]
         _lib = L
     return _lib
@@ -462,3 +464,19 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def stat_moments_device_noalloc(d_wav, J, B, N, alpha, d_out, d_mask=None, stream=None):
+    """Per-scale moments of packed device coefficients without any allocation
+    (s2dw_hip_stat_moments_device_noalloc): d_wav, d_mask (or None) and d_out
+    (8 x (J+1) float64) are torch tensors on the current device; enqueued on
+    `stream` (default: torch's current stream)."""
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    _check(lib().s2dw_hip_stat_moments_device_noalloc(
+        C.c_void_p(d_wav.data_ptr()), C.c_void_p(d_mask.data_ptr()) if d_mask is not None else None, int(J), int(B),
+        int(N), float(alpha), C.c_void_p(d_out.data_ptr()), C.c_void_p(stream.cuda_stream)),
+        "s2dw_hip_stat_moments_device_noalloc")
+    return d_out

@@ -421,6 +421,11 @@ int s2dw_hip_analysis_moments_device(s2dw_hip_plan_t plan, double *d_out, double
 
 int s2dw_hip_plan_moments_fusable(s2dw_hip_plan_t plan) { return plan && plan->p.momf_ok ? 1 : 0; }
 
+int s2dw_hip_stat_moments_device_noalloc(double *d_wav, const double *d_mask, int J, int B, int N, double alpha,
+                                         double *d_out, void *stream) {
+  return moments_noalloc(d_wav, d_mask, J, B, N, alpha, d_out, static_cast<hipStream_t>(stream));
+}
+
 namespace {
 struct DevBufs {
   std::vector<void *> v;

@@ -233,6 +233,10 @@ int moments_run(hipStream_t st, const long long *off, const long long *count, in
 // values) owns records rec0[s] .. rec0[s+1]-1 (8 doubles each).
 int moments_finish(hipStream_t st, const long long *count, const long long *rec0, int nscales, const double *d_part,
                    double *d_out);
+// Moments of packed coefficients of (J, B, N, alpha) with the records in the
+// library's static device scratch (no allocation).
+int moments_noalloc(double *d_wav, const double *d_mask, int J, int B, int N, double alpha, double *d_out,
+                    hipStream_t st);
 int wigner_plane(int l, double *out, double *out2);
 // Reference-rounding Wigner planes (s2dw_planes.hip): doubles of the table for
 // band limit L, and its construction into d_Sa with the exact thin columns.

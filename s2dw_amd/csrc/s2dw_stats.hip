@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
+#include <mutex>
 
 #include "s2dw_internal.h"
 #include "s2dw_moments.h"
@@ -229,6 +231,68 @@ int moments_finish(hipStream_t st, const long long *count, const long long *rec0
     S2DW_HIP_CHECK(hipGetLastError());
   }
   return kErrNone;
+}
+
+// ---------------------------------------------------------------------------
+// Moments without allocation (s2dw_hip_stat_moments_device_noalloc): the
+// chunk records live in a static device array, `scratch`, of kScratchDoubles
+// doubles (1 MiB) that every GPU the library runs on holds until the process
+// ends.  Host code never passes the array's name to a kernel: on the first
+// call on a device (that device current) hipGetSymbolAddress gives the
+// array's address on that device, kept per device; calls use that address.
+// The records are written before they are read, so nothing is cleared; calls
+// that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr long long kScratchDoubles = 131072;
+constexpr int kScratchDevices = 64;
+constexpr int kScratchScales = 32;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+double *g_scratch[kScratchDevices] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+
+int moments_noalloc(double *d_wav, const double *d_mask, int J, int B, int N, double alpha, double *d_out,
+                    hipStream_t st) {
+  if (!d_wav || !d_out || J < 0 || B < 1 || N < 1 || !(alpha > 1.0)) return kErrArgInvalid;
+  if (J + 1 > kScratchScales) {
+    set_error("s2dw_hip_stat_moments_device_noalloc: J=%d exceeds the %d scales it handles", J, kScratchScales - 1);
+    return kErrSizeInvalid;
+  }
+  if (((uintptr_t)d_wav & 15) || ((uintptr_t)d_mask & 15)) {
+    set_error("s2dw_hip_stat_moments_device_noalloc: d_wav and d_mask must be 16-byte aligned");
+    return kErrArgInvalid;
+  }
+  const int n = J + 1;
+  long long off[kScratchScales], cnt[kScratchScales], total = 0;
+  for (int j = 0; j < n; ++j) {
+    int hi = 0, lo = 0;
+    band_limits(j, B, alpha, &hi, &lo);
+    cnt[j] = (long long)(2 * hi - 1) * 2 * hi * N;
+    off[j] = total;
+    total += cnt[j];
+  }
+  const long long need = moments_partial_doubles(cnt, n);
+  if (need > kScratchDoubles) {
+    set_error("s2dw_hip_stat_moments_device_noalloc: B=%d N=%d J=%d needs %lld doubles of scratch, more than the "
+              "%lld it holds",
+              B, N, J, need, kScratchDoubles);
+    return kErrSizeInvalid;
+  }
+  int dev = 0;
+  S2DW_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) return kErrInitFail;
+  double *ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[dev]) {
+      void *q = nullptr;
+      S2DW_HIP_CHECK(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+      g_scratch[dev] = static_cast<double *>(q);
+    }
+    ptr = g_scratch[dev];
+  }
+  return moments_run(st, off, cnt, n, d_wav, d_mask, ptr, d_out);
 }
 
 }  // namespace s2dw
