@@ -247,9 +247,10 @@ int s2dw_hip_plan_stage_times(s2dw_hip_plan_t plan, double *ms, int *count);
  * wavdyn[jj]: (2hi-1)*2hi*N doubles (masked in place when mask != NULL);
  * mask: NULL or mask[jj] of the same shapes; mean..kur: J+1 doubles each;
  * five, six: J+1 doubles or NULL (the optional outputs).  Returns
- * S2DW_ERROR_SIZE_INVALID if a scale has nj < 2 (the reference halts),
- * S2DW_ERROR_ARTH_WARNING (results complete) if a variance is below
- * TOL_ZERO. */
+ * S2DW_ERROR_SIZE_INVALID if a scale has nj < 2 (the reference halts there:
+ * the outputs of the scales before it are filled and var of that scale is
+ * NaN), S2DW_ERROR_ARTH_WARNING (results complete) if a variance is below
+ * TOL_ZERO (the reference warns once per such scale: var tells which). */
 int s2dw_hip_stat_moments(double *const *wavdyn, int J, int B, int N, double alpha, double *mean, double *var,
                           double *skew, double *kur, const double *const *mask, double *five, double *six);
 

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <list>
 #include <memory>
 #include <mutex>
@@ -486,6 +487,9 @@ int s2dw_hip_stat_moments(double *const *wavdyn, int J, int B, int N, double alp
   int warn = -1;
   for (int j = 0; j < n; ++j) {
     if (out[7 * (size_t)n + j] == 2) {
+      // the scales before it are filled; its variance is NaN (the Fortran
+      // module warns for the earlier zero-variance scales, then halts here)
+      var[j] = std::numeric_limits<double>::quiet_NaN();
       set_error("s2dw_stat_moments: scale %d has %.0f samples; at least two are required", j, out[6 * (size_t)n + j]);
       return kErrSizeInvalid;
     }

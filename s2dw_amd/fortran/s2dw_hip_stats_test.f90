@@ -3,7 +3,8 @@
 !
 ! Usage: s2dw_hip_stats_test B N alpha mode
 !   mode 0: no mask; 1: a mask of zeros and ones; 2: an all-zero mask on
-!   scale 0 (the routine halts as the reference does).
+!   scale 0 (the routine halts as the reference does); 3: no mask, scales 1
+!   and 2 constant (zero variance: one warning each, as the reference).
 ! Usage: s2dw_hip_stats_test write nsim J nopt out
 !   s2dw_stat_moments_write into the text file out, for the values
 !   v(isim, jj, k) = (-1)**k * (isim + 1.5) * 10**(jj - 3) / (k + 3)
@@ -88,7 +89,11 @@ program s2dw_hip_stats_test
      end do
   end do
   if (mode == 2) mask(0)%coeff = 0d0
-  if (mode == 0) then
+  if (mode == 3) then
+     wavdyn(1)%coeff = 0.25d0
+     wavdyn(2)%coeff = -1.5d0
+  end if
+  if (mode == 0 .or. mode == 3) then
      call s2dw_stat_moments(wavdyn, J, B, N, alpha, mean, var, skew, kur, five=five, six=six)
   else
      call s2dw_stat_moments(wavdyn, J, B, N, alpha, mean, var, skew, kur, mask, five, six)

@@ -16,6 +16,7 @@
 module s2dw_stat_mod
 
   use, intrinsic :: iso_c_binding
+  use, intrinsic :: ieee_arithmetic, only: ieee_is_nan
   use s2dw_types_mod
   use s2dw_error_mod
   use s2dw_core_mod
@@ -78,12 +79,16 @@ contains
     code = c_stat_moments(ptrs, int(J, c_int), int(B, c_int), int(N, c_int), real(alpha, c_double), &
          mean, var, skew, kur, mp, fp, sp)
     select case (code)
-    case (0)
-    case (S2DW_ERROR_ARTH_WARNING)
-       call s2dw_error(S2DW_ERROR_ARTH_WARNING, 's2dw_stat_moments', &
-            'Skewness/kurtosis & higher undefined when variance zero')
-    case (S2DW_ERROR_SIZE_INVALID)
-       call s2dw_error(S2DW_ERROR_SIZE_INVALID, 's2dw_stat_moments', &
+    case (0, S2DW_ERROR_ARTH_WARNING, S2DW_ERROR_SIZE_INVALID)
+       ! as the reference's scale loop: a warning for each zero-variance scale
+       ! in scale order, and the halt at the first scale with fewer than two
+       ! samples (its variance comes back NaN; later scales are not computed)
+       do jj = 0, J
+          if (ieee_is_nan(var(jj))) exit
+          if (var(jj) < TOL_ZERO) call s2dw_error(S2DW_ERROR_ARTH_WARNING, 's2dw_stat_moments', &
+               'Skewness/kurtosis & higher undefined when variance zero')
+       end do
+       if (code == S2DW_ERROR_SIZE_INVALID) call s2dw_error(S2DW_ERROR_SIZE_INVALID, 's2dw_stat_moments', &
             comment_add='Require at least two samples to compute moments')
     case default
        call s2dw_error(int(code), 's2dw_stat_moments')

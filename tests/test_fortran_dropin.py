@@ -153,3 +153,17 @@ def test_fortran_stat_moments_halts_like_reference():
     r = run([STATS_TEST, "16", "2", "2.0", "2"])
     assert "Require at least two samples to compute moments" in r.stdout
     assert "Halting program execution" in r.stdout
+
+
+@pytest.mark.gpu
+def test_fortran_stat_moments_warns_per_zero_variance_scale():
+    # scales 1 and 2 constant: the reference's loop warns once for each
+    # (s2dw_stat_mod.f90:251-253) and still returns every scale's moments
+    B, N, alpha = 32, 3, 2.0
+    r = run([STATS_TEST, str(B), str(N), str(alpha), "3"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("Skewness/kurtosis & higher undefined when variance zero") == 2
+    rows = [line.split() for line in r.stdout.strip().splitlines() if line.split() and line.split()[0].isdigit()]
+    assert len(rows) == 6
+    assert abs(float(rows[1][2])) < 1e-12 and abs(float(rows[2][2])) < 1e-12  # variance
+    assert float(rows[0][2]) > 0.0
