@@ -1,16 +1,313 @@
 !------------------------------------------------------------------------------
-! s2dw_fileio_mod -- present so that host programs which `use` the module
-! compile against this library.  The FITS and MATLAB wavelet-file readers and
-! writers of the reference (src/mod/s2dw_fileio_mod.f90) are outside the
-! transform path this library accelerates and are not provided.
+! s2dw_fileio_mod -- the S2DW wavelet-coefficient file format (FITS) for
+! programs that use this library in place of the reference.
+!
+! Public interface as the reference's module (src/mod/s2dw_fileio_mod.f90):
+! the generic s2dw_fileio_fits_wav_write / _read, each for statically
+! allocated coefficients wav(0:J, 0:2B-2, 0:2B-1, 0:N-1) and for the
+! per-scale type s2dw_wav_abg.  The file layout is the reference's, so files
+! are interchangeable both ways:
+!   HDU 1  empty primary image; comment cards, DATE, the optional comment and
+!          the keys J, B, N, BLSCOEFF (integers) and ALPHA (8 decimals)
+!          (reference 845-862)
+!   HDU 2  SCOEFF_REAL, bl_scoeff x bl_scoeff FP64 image (871-875)
+!   HDU 3  SCOEFF_IMAG, likewise (876-879)
+!   HDU 4+jj  WAV, the (2hi-1) x 2hi x N FP64 cube of scale jj (900-915)
+! Writing refuses an existing file (S2DW_ERROR_FILEIO, then stop), reading
+! checks the number of HDUs, the scaling-coefficient sizes and for null
+! values, as the reference does (1140-1240).  I/O goes through CFITSIO's
+! Fortran interface (link with -lcfitsio).
+! The MATLAB text writers and readers of the reference (s2dw_fileio_mod.f90:
+! 102-763) are not provided.
 !------------------------------------------------------------------------------
 module s2dw_fileio_mod
 
   use s2dw_types_mod
+  use s2dw_error_mod
   use s2dw_core_mod
 
   implicit none
 
   private
+
+  public :: &
+       s2dw_fileio_fits_wav_write, &
+       s2dw_fileio_fits_wav_read
+
+  interface s2dw_fileio_fits_wav_write
+     module procedure fits_write_static, fits_write_dynamic
+  end interface s2dw_fileio_fits_wav_write
+
+  interface s2dw_fileio_fits_wav_read
+     module procedure fits_read_static, fits_read_dynamic
+  end interface s2dw_fileio_fits_wav_read
+
+  integer, parameter :: BITPIX_DP = -64     ! FP64 images
+  integer, parameter :: ALPHA_DECIMALS = 8  ! digits of ALPHA in the header
+  integer, parameter :: FITS_NO_FILE = 104  ! CFITSIO: could not open the named file
+
+contains
+
+  ! Band limit bl_hi of scale jj, as the transform computes it
+  ! (s2dw_core_mod.f90:606-611).
+  integer function scale_hi(jj, B, alpha) result(hi)
+    integer, intent(in) :: jj, B
+    real(dp), intent(in) :: alpha
+    if (s2dw_core_assimilate_int(B / (alpha**(jj-1)), hi)) then
+       hi = min(hi, B)
+    else
+       hi = min(ceiling(B / (alpha**(jj-1))), B)
+    end if
+  end function scale_hi
+
+  ! Reports a CFITSIO failure with its messages and halts when asked.
+  subroutine fits_fail(status, halt)
+    integer, intent(inout) :: status
+    logical, intent(in) :: halt
+    character(len=30) :: text
+    character(len=80) :: msg
+    if (status <= 0) return
+    call ftgerr(status, text)
+    print *, 'FITSIO Error Status =', status, ': ', text
+    do
+       call ftgmsg(msg)
+       if (msg == ' ') exit
+       write(*,*) trim(msg)
+    end do
+    if (halt) stop
+  end subroutine fits_fail
+
+  ! True when filename can be opened (or exists but fails to open).
+  logical function fits_present(filename) result(exists)
+    character(len=*), intent(in) :: filename
+    integer :: unit, status, blocksize
+    status = 0
+    call ftgiou(unit, status)
+    call ftopen(unit, filename, 0, blocksize, status)
+    if (status == 0) then
+       exists = .true.
+       call ftclos(unit, status)
+    else if (status == FITS_NO_FILE) then
+       exists = .false.
+       status = 0
+       call ftcmsg()
+    else
+       exists = .true.
+       call fits_fail(status, .false.)
+       status = 0
+       call ftclos(unit, status)
+       status = 0
+    end if
+    call ftfiou(unit, status)
+  end function fits_present
+
+  ! Creates filename (which must not exist) with the primary header and the
+  ! scaling coefficients; returns the open unit.
+  subroutine fits_create(filename, proc, J, B, N, bl_scoeff, alpha, scoeff, comment, unit, status)
+    character(len=*), intent(in) :: filename, proc
+    integer, intent(in) :: J, B, N, bl_scoeff
+    real(dp), intent(in) :: alpha
+    complex(dpc), intent(in) :: scoeff(0:bl_scoeff-1, 0:bl_scoeff-1)
+    character(len=*), intent(in), optional :: comment
+    integer, intent(out) :: unit
+    integer, intent(inout) :: status
+    integer :: naxes(2)
+    real(dp), allocatable :: part(:,:)
+    if (fits_present(filename)) then
+       call s2dw_error(S2DW_ERROR_FILEIO, proc, comment_add='File already exists')
+       stop
+    end if
+    call ftgiou(unit, status)
+    call ftinit(unit, filename, 1, status)
+    naxes = 0
+    call ftphpr(unit, .true., BITPIX_DP, 0, naxes, 0, 1, .true., status)
+    call ftpcom(unit, '  Scale discretised wavelet coefficients created by S2DW code', status)
+    call ftpcom(unit, '  Written by Jason McEwen (mcewen@mrao.cam.ac.uk)', status)
+    call ftpcom(unit, '  Primary extension empty.', status)
+    call ftpdat(unit, status)
+    if (present(comment)) call ftpcom(unit, comment, status)
+    call ftpkyj(unit, 'J', J, 'No. of scales', status)
+    call ftpkyj(unit, 'B', B, 'Band limit', status)
+    call ftpkyj(unit, 'N', N, 'Azimuthal band limit', status)
+    call ftpkyj(unit, 'BLSCOEFF', bl_scoeff, 'Scaling coefficient band limit', status)
+    call ftpkyd(unit, 'ALPHA', alpha, ALPHA_DECIMALS, 'Alpha', status)
+    allocate(part(0:bl_scoeff-1, 0:bl_scoeff-1))
+    naxes = bl_scoeff
+    part = real(scoeff, dp)
+    call ftiimg(unit, BITPIX_DP, 2, naxes, status)
+    call ftpkys(unit, 'EXTNAME', 'SCOEFF_REAL', 'entension name', status)
+    call ftp2dd(unit, 1, bl_scoeff, bl_scoeff, bl_scoeff, part, status)
+    part = aimag(scoeff)
+    call ftiimg(unit, BITPIX_DP, 2, naxes, status)
+    call ftpkys(unit, 'EXTNAME', 'SCOEFF_IMAG', 'entension name', status)
+    call ftp2dd(unit, 1, bl_scoeff, bl_scoeff, bl_scoeff, part, status)
+  end subroutine fits_create
+
+  ! Appends one WAV cube (n1 x n2 x n3, contiguous).
+  subroutine fits_put_cube(unit, cube, n1, n2, n3, status)
+    integer, intent(in) :: unit, n1, n2, n3
+    real(dp), intent(in) :: cube(n1, n2, n3)
+    integer, intent(inout) :: status
+    integer :: naxes(3)
+    naxes = [n1, n2, n3]
+    call ftiimg(unit, BITPIX_DP, 3, naxes, status)
+    call ftpkys(unit, 'EXTNAME', 'WAV', 'entension name', status)
+    call ftp3dd(unit, 1, n1, n2, n1, n2, n3, cube, status)
+  end subroutine fits_put_cube
+
+  subroutine fits_finish(unit, status)
+    integer, intent(in) :: unit
+    integer, intent(inout) :: status
+    call ftclos(unit, status)
+    call ftfiou(unit, status)
+    if (status > 0) call fits_fail(status, .true.)
+  end subroutine fits_finish
+
+  ! s2dw_fileio_fits_wav_write_static, reference 796-926.
+  subroutine fits_write_static(wav, scoeff, J, B, N, bl_scoeff, alpha, filename, comment)
+    integer, intent(in) :: J, B, N, bl_scoeff
+    real(dp), intent(in) :: alpha
+    real(dp), intent(in) :: wav(0:J, 0:2*B-2, 0:2*B-1, 0:N-1)
+    complex(dpc), intent(in) :: scoeff(0:bl_scoeff-1, 0:bl_scoeff-1)
+    character(len=*), intent(in) :: filename
+    character(len=*), intent(in), optional :: comment
+    integer :: unit, status, jj, hi
+    real(dp), allocatable :: cube(:,:,:)
+    status = 0
+    call fits_create(filename, 's2dw_fileio_fits_wav_write', J, B, N, bl_scoeff, alpha, scoeff, comment, unit, status)
+    do jj = 0, J
+       hi = scale_hi(jj, B, alpha)
+       cube = wav(jj, 0:2*hi-2, 0:2*hi-1, 0:N-1)
+       call fits_put_cube(unit, cube, 2*hi-1, 2*hi, N, status)
+    end do
+    call fits_finish(unit, status)
+  end subroutine fits_write_static
+
+  ! s2dw_fileio_fits_wav_write_dynamic, reference 956-1077.
+  subroutine fits_write_dynamic(wavdyn, scoeff, J, B, N, bl_scoeff, alpha, filename, comment)
+    integer, intent(in) :: J, B, N, bl_scoeff
+    real(dp), intent(in) :: alpha
+    type(s2dw_wav_abg), intent(in), allocatable :: wavdyn(:)
+    complex(dpc), intent(in) :: scoeff(0:bl_scoeff-1, 0:bl_scoeff-1)
+    character(len=*), intent(in) :: filename
+    character(len=*), intent(in), optional :: comment
+    integer :: unit, status, jj
+    status = 0
+    call fits_create(filename, 's2dw_fileio_fits_wav_write_dynamic', J, B, N, bl_scoeff, alpha, scoeff, comment, &
+         unit, status)
+    do jj = 0, J
+       call fits_put_cube(unit, wavdyn(jj)%coeff, size(wavdyn(jj)%coeff, 1), size(wavdyn(jj)%coeff, 2), &
+            size(wavdyn(jj)%coeff, 3), status)
+    end do
+    call fits_finish(unit, status)
+  end subroutine fits_write_dynamic
+
+  ! Opens filename, reads the primary keys and the scaling coefficients and
+  ! checks the HDU count; leaves the unit at HDU 3.
+  subroutine fits_open_read(filename, proc, J, B, N, bl_scoeff, alpha, scoeff, unit, status)
+    character(len=*), intent(in) :: filename, proc
+    integer, intent(out) :: J, B, N, bl_scoeff
+    real(dp), intent(out) :: alpha
+    complex(dpc), allocatable, intent(out) :: scoeff(:,:)
+    integer, intent(out) :: unit
+    integer, intent(inout) :: status
+    character(len=20) :: note
+    integer :: blocksize, nhdu_file, hdutype, n1, n2, part, fail
+    logical :: anynull
+    real(dp), allocatable :: re(:,:)
+    if (.not. fits_present(filename)) &
+         call s2dw_error(S2DW_ERROR_FILEIO, proc, comment_add='File does not exist')
+    call ftgiou(unit, status)
+    call ftopen(unit, filename, 0, blocksize, status)
+    call ftgkyj(unit, 'J', J, note, status)
+    call ftgkyj(unit, 'B', B, note, status)
+    call ftgkyj(unit, 'N', N, note, status)
+    call ftgkyj(unit, 'BLSCOEFF', bl_scoeff, note, status)
+    call ftgkyd(unit, 'ALPHA', alpha, note, status)
+    ! primary, two scaling-coefficient images, one cube per scale
+    call ftthdu(unit, nhdu_file, status)
+    if (nhdu_file /= 3 + (J + 1)) call s2dw_error(S2DW_ERROR_FILEIO, proc, comment_add='Invalid number of headers')
+    fail = 0
+    allocate(scoeff(0:bl_scoeff-1, 0:bl_scoeff-1), re(0:bl_scoeff-1, 0:bl_scoeff-1), stat=fail)
+    if (fail /= 0) call s2dw_error(S2DW_ERROR_MEM_ALLOC_FAIL, proc)
+    scoeff = cmplx(0d0, 0d0, dpc)
+    do part = 0, 1
+       call ftmahd(unit, 2 + part, hdutype, status)
+       call ftgkyj(unit, 'NAXIS1', n1, note, status)
+       call ftgkyj(unit, 'NAXIS2', n2, note, status)
+       if (n1 /= bl_scoeff .or. n2 /= bl_scoeff) &
+            call s2dw_error(S2DW_ERROR_FILEIO, proc, comment_add='Inconsistent scaling coefficients sizes in file')
+       call ftg2dd(unit, 1, -999d0, bl_scoeff, bl_scoeff, bl_scoeff, re, anynull, status)
+       if (part == 0) then
+          scoeff = cmplx(re, 0d0, dpc)
+       else
+          scoeff = scoeff + I * re
+       end if
+    end do
+  end subroutine fits_open_read
+
+  ! Reads the cube of HDU 4 + jj into cube (allocated to its NAXIS sizes).
+  subroutine fits_get_cube(unit, jj, proc, cube, status)
+    integer, intent(in) :: unit, jj
+    character(len=*), intent(in) :: proc
+    real(dp), allocatable, intent(out) :: cube(:,:,:)
+    integer, intent(inout) :: status
+    character(len=20) :: note
+    integer :: hdutype, n1, n2, n3, fail
+    logical :: anynull
+    call ftmahd(unit, 4 + jj, hdutype, status)
+    call ftgkyj(unit, 'NAXIS1', n1, note, status)
+    call ftgkyj(unit, 'NAXIS2', n2, note, status)
+    call ftgkyj(unit, 'NAXIS3', n3, note, status)
+    fail = 0
+    allocate(cube(0:n1-1, 0:n2-1, 0:n3-1), stat=fail)
+    if (fail /= 0) call s2dw_error(S2DW_ERROR_MEM_ALLOC_FAIL, proc)
+    cube = 0d0
+    anynull = .false.
+    call ftg3dd(unit, 1, -999d0, n1, n2, n1, n2, n3, cube, anynull, status)
+    if (anynull) call s2dw_error(S2DW_ERROR_FILEIO, proc, comment_add='Null wavelet coefficients read from file')
+  end subroutine fits_get_cube
+
+  ! s2dw_fileio_fits_wav_read_static, reference 1109-1253: wav allocated
+  ! (0:J, 0:2B-2, 0:2B-1, 0:N-1), zero outside each scale's block.
+  subroutine fits_read_static(wav, scoeff, J, B, N, bl_scoeff, alpha, filename)
+    integer, intent(out) :: J, B, N, bl_scoeff
+    real(dp), intent(out) :: alpha
+    real(dp), allocatable, intent(out) :: wav(:,:,:,:)
+    complex(dpc), allocatable, intent(out) :: scoeff(:,:)
+    character(len=*), intent(in) :: filename
+    integer :: unit, status, jj, fail
+    real(dp), allocatable :: cube(:,:,:)
+    status = 0
+    call fits_open_read(filename, 's2dw_fileio_fits_wav_read', J, B, N, bl_scoeff, alpha, scoeff, unit, status)
+    fail = 0
+    allocate(wav(0:J, 0:2*B-2, 0:2*B-1, 0:N-1), stat=fail)
+    if (fail /= 0) call s2dw_error(S2DW_ERROR_MEM_ALLOC_FAIL, 's2dw_fileio_fits_wav_read')
+    wav = 0d0
+    do jj = 0, J
+       call fits_get_cube(unit, jj, 's2dw_fileio_fits_wav_read', cube, status)
+       wav(jj, 0:size(cube, 1)-1, 0:size(cube, 2)-1, 0:size(cube, 3)-1) = cube
+    end do
+    call fits_finish(unit, status)
+  end subroutine fits_read_static
+
+  ! s2dw_fileio_fits_wav_read_dynamic, reference 1286-1437.
+  subroutine fits_read_dynamic(wavdyn, scoeff, J, B, N, bl_scoeff, alpha, filename)
+    integer, intent(out) :: J, B, N, bl_scoeff
+    real(dp), intent(out) :: alpha
+    type(s2dw_wav_abg), intent(out), allocatable :: wavdyn(:)
+    complex(dpc), allocatable, intent(out) :: scoeff(:,:)
+    character(len=*), intent(in) :: filename
+    integer :: unit, status, jj, fail
+    status = 0
+    call fits_open_read(filename, 's2dw_fileio_fits_wav_read', J, B, N, bl_scoeff, alpha, scoeff, unit, status)
+    fail = 0
+    allocate(wavdyn(0:J), stat=fail)
+    if (fail /= 0) call s2dw_error(S2DW_ERROR_MEM_ALLOC_FAIL, 's2dw_fileio_fits_wav_read_dynamic')
+    do jj = 0, J
+       call fits_get_cube(unit, jj, 's2dw_fileio_fits_wav_read_dynamic', wavdyn(jj)%coeff, status)
+    end do
+    call fits_finish(unit, status)
+  end subroutine fits_read_dynamic
 
 end module s2dw_fileio_mod

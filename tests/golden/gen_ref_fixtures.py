@@ -6,7 +6,11 @@ stores its outputs as
 small .npz fixtures beside this script.  Run in a container that holds the
 reference sources; the fixtures are committed, the GPU box only reads them.
 
-    python tests/golden/gen_ref_fixtures.py [--moments-only]
+    python tests/golden/gen_ref_fixtures.py [--moments-only | --fits-only]
+
+The FITS fixtures are wavelet files written by the reference's
+s2dw_fileio_fits_wav_write (oracle/_ref/ref_fits: s2dw_amd/fortran/
+s2dw_hip_fits_test.f90 linked against the reference's s2dw_fileio_mod).
 """
 from __future__ import annotations
 
@@ -20,6 +24,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+REF_FITS = os.path.join(ROOT, "oracle", "_ref", "ref_fits")
+# (layout, B, N, alpha): the wavelet files written by the reference
+FITS_CASES = [("dynamic", 8, 2, 2.0), ("static", 12, 3, 3.0)]
 
 # (B, alpha, J); J=None means J_max.  Covers the test config (64,2,full J),
 # L=256 N=5 full J, L=512 J_min=2 (J=7), alpha=3 and non-power-of-alpha B.
@@ -103,13 +110,31 @@ def moments_fixtures():
             fh.write(run("momwrite", 3, 4, nopt))
 
 
+def fits_name(layout, B, N, alpha):
+    return f"ref_wav_{layout}_B{B}_N{N}_a{int(alpha)}.fits"
+
+
+def fits_fixtures():
+    for layout, B, N, alpha in FITS_CASES:
+        out = os.path.join(HERE, fits_name(layout, B, N, alpha))
+        if os.path.exists(out):
+            os.remove(out)  # the writer refuses an existing file
+        subprocess.run([REF_FITS, "write_" + layout, out, str(B), str(N), str(alpha)], check=True,
+                       capture_output=True)
+
+
 def main():
-    if not os.path.exists(REF_DUMP):
-        raise SystemExit("build the reference driver first: make -C oracle ref")
+    if not os.path.exists(REF_DUMP) or not os.path.exists(REF_FITS):
+        raise SystemExit("build the reference drivers first: make -C oracle ref")
+    if "--fits-only" in sys.argv:
+        fits_fixtures()
+        print("fixtures written to", HERE)
+        return
     if "--moments-only" in sys.argv:
         moments_fixtures()
         print("fixtures written to", HERE)
         return
+    fits_fixtures()
     moments_fixtures()
     # --- kernels / admissibility / band limits
     kern = {}
