@@ -59,6 +59,40 @@ struct RadixConst<7> {
                                   -0.43388373911755812048, -0.97492791218182360702, -0.78183148246802980871};
 };
 
+// cos/sin(2 pi e / R), e = 1..R-1 (entry e-1), to 21 decimals
+template <>
+struct RadixConst<11> {
+  static constexpr double c[10] = {0.841253532831181168862, 0.415415013001886425529, -0.142314838273285140444,
+                                   -0.654860733945285064057, -0.959492973614497389890, -0.959492973614497389890,
+                                   -0.654860733945285064057, -0.142314838273285140444, 0.415415013001886425529,
+                                   0.841253532831181168862};
+  static constexpr double s[10] = {0.540640817455597582108, 0.909631995354518371412, 0.989821441880932732376,
+                                   0.755749574354258283774, 0.281732556841429697711, -0.281732556841429697711,
+                                   -0.755749574354258283774, -0.989821441880932732376, -0.909631995354518371412,
+                                   -0.540640817455597582108};
+};
+template <>
+struct RadixConst<31> {
+  static constexpr double c[30] = {
+      0.979529941252494493938,  0.918957811620230629127,  0.820763441207276326364,  0.688966919075686567801,
+      0.528964010326962457365,  0.347305252844820285542,  0.151427777504576663657,  -0.050649168838712712279,
+      -0.250652532258720539315, -0.440394151557634309516, -0.612105982547662844147, -0.758758122692790901913,
+      -0.874346616144582118827, -0.954139256400048851476, -0.994869323391895146321, -0.994869323391895146321,
+      -0.954139256400048851476, -0.874346616144582118827, -0.758758122692790901913, -0.612105982547662844147,
+      -0.440394151557634309516, -0.250652532258720539315, -0.050649168838712712279, 0.151427777504576663657,
+      0.347305252844820285542,  0.528964010326962457365,  0.688966919075686567801,  0.820763441207276326364,
+      0.918957811620230629127,  0.979529941252494493938};
+  static constexpr double s[30] = {
+      0.201298520088660079142,  0.394355855113318580102,  0.571268215094792279157,  0.724792787229119958865,
+      0.848644257494750950464,  0.937752132147080458429,  0.988468324328111399162,  0.998716507171052807146,
+      0.968077118866204305153,  0.897804539570741657137,  0.790775736937698582078,  0.651372482722222207454,
+      0.485301962531081025215,  0.299363122973357954008,  0.101168321987432177786,  -0.101168321987432177786,
+      -0.299363122973357954008, -0.485301962531081025215, -0.651372482722222207454, -0.790775736937698582078,
+      -0.897804539570741657137, -0.968077118866204305153, -0.998716507171052807146, -0.988468324328111399162,
+      -0.937752132147080458429, -0.848644257494750950464, -0.724792787229119958865, -0.571268215094792279157,
+      -0.394355855113318580102, -0.201298520088660079142};
+};
+
 // In-register DFT of odd prime length R using the symmetric pairs
 // (x_r + x_{R-r}, x_r - x_{R-r}).
 template <int R, int DIR>
@@ -367,6 +401,16 @@ __device__ __forceinline__ void wave_fft_exec(const FftSpec &s, double2 *buf, in
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lpad(int i) { return i ^ ((i >> 3) & 15); }
 
+// threadIdx.x as a value the compiler cannot trace back to the thread index:
+// each stage then computes its slot indices where it uses them, instead of
+// having them hoisted out of a tile loop around the transform and held in
+// registers through every stage.
+__device__ __forceinline__ int stage_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 // Workgroup barrier that orders LDS only: the fences are restricted to the
 // local address space, so outstanding global loads (prefetches) stay in
 // flight across it.  Used where the barrier protects LDS data alone.
@@ -412,9 +456,10 @@ __device__ __forceinline__ void team_stage(double2 *buf, int M, int logTw, int N
   double2 w1[NB], w2[NB], w4[NB], w8[NB];
   double2 pm[NB][R];
   int tt[NB], idxD[NB];
+  const int tid = stage_tid();
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
-    const int b = threadIdx.x + NT * q;
+    const int b = tid + NT * q;
     tt[q] = b & (Tw - 1);
     const int jj = b >> logTw;
     const int jm = jj & (Ns - 1);
@@ -459,9 +504,10 @@ __device__ __forceinline__ void team_stage8(double2 *buf, int M, int logTw, int 
   const int nR = M / R;
   double2 v[NB][R];
   int tt[NB], idxD[NB];
+  const int tid = stage_tid();
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
-    const int b = threadIdx.x + NT * q;
+    const int b = tid + NT * q;
     tt[q] = b & (Tw - 1);
     const int jj = b >> logTw;
     const int jm = jj & (Ns - 1);
@@ -721,6 +767,99 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
   const int q = nb / X, rm = nb % X;
   const int x = bid % X, i = bid / X;
   return x * q + (x < rm ? x : rm) + i;
+}
+
+// ---------------------------------------------------------------------------
+// Odd-length workgroup engine: length NN = R0 R1 R2 with odd prime radices,
+// T sequences interleaved in LDS (element k of sequence t at slot k T + t),
+// NT threads.  Used where Bluestein would double the work length (2hi - 1 =
+// 1023 = 31 * 11 * 3 runs here instead of as two length-2048 transforms).
+//
+// Stockham stages in the order R0, R1, R2 (Ns = 1, R0, R0 R1).  A stage
+// reads every butterfly of the thread into registers (and, for Ns > 1, its
+// twiddles w^{r jm} from a table of e^{+2 pi i k/NN}, one rounding each),
+// waits at a barrier, transforms and writes in place; a second barrier
+// closes the stage.  The radix-R butterfly forms the symmetric pairs
+// x_r +- x_{R-r} in place and writes each output pair y_q, y_{R-q} as soon
+// as it is accumulated, so a thread holds its R inputs and four
+// accumulators rather than 2R values.
+// ---------------------------------------------------------------------------
+template <int R, int DIR, typename Out>
+__device__ __forceinline__ void dft_odd_out(double2 *v, Out out) {
+  constexpr int H = (R - 1) / 2;
+#pragma unroll
+  for (int r = 1; r <= H; ++r) {
+    const double2 a = v[r], b = v[R - r];
+    v[r] = cadd(a, b);
+    v[R - r] = csub(a, b);
+  }
+  double2 y0 = v[0];
+#pragma unroll
+  for (int r = 1; r <= H; ++r) y0 = cadd(y0, v[r]);
+  out(0, y0);
+#pragma unroll
+  for (int q = 1; q <= H; ++q) {
+    double re = v[0].x, im = v[0].y, ire = 0.0, iim = 0.0;
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      const int e = (r * q) % R;
+      const double cc = RadixConst<R>::c[e - 1], ss = RadixConst<R>::s[e - 1];
+      re += v[r].x * cc;
+      im += v[r].y * cc;
+      ire += v[R - r].x * ss;
+      iim += v[R - r].y * ss;
+    }
+    const double2 base = make_double2(re, im);
+    const double2 rot = cmuli<DIR>(make_double2(ire, iim));
+    out(q, cadd(base, rot));
+    out(R - q, csub(base, rot));
+  }
+}
+
+template <int R, int NN, int NS, int DIR, int NT, int T>
+__device__ __forceinline__ void odd_stage(double2 *buf, const double2 *__restrict__ tw) {
+  constexpr int nR = NN / R, nb = nR * T, NB = (nb + NT - 1) / NT, twstep = NN / (NS * R);
+  const int tid = stage_tid();
+  double2 v[NB][R];
+  double2 w[NB][R];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int b = tid + NT * q;
+    if (b < nb) {
+      const int t = b % T, jj = b / T;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = buf[(jj + r * nR) * T + t];
+      if constexpr (NS > 1) {
+        const int jm = jj % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[q][r] = tw[r * jm * twstep];
+      }
+    }
+  }
+  lds_barrier();
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int b = tid + NT * q;
+    if (b < nb) {
+      const int t = b % T, jj = b / T;
+      const int jm = NS > 1 ? jj % NS : 0;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q][r] = cmulw<DIR>(v[q][r], w[q][r]);
+      }
+      const int o = ((jj - jm) * R + jm) * T + t;
+      dft_odd_out<R, DIR>(v[q], [&](int r, double2 y) { buf[o + r * NS * T] = y; });
+    }
+  }
+  lds_barrier();
+}
+
+template <int R0, int R1, int R2, int DIR, int NT, int T>
+__device__ __forceinline__ void odd_fft(double2 *buf, const double2 *__restrict__ tw) {
+  constexpr int NN = R0 * R1 * R2;
+  odd_stage<R0, NN, 1, DIR, NT, T>(buf, tw);
+  odd_stage<R1, NN, R0, DIR, NT, T>(buf, tw);
+  odd_stage<R2, NN, R0 * R1, DIR, NT, T>(buf, tw);
 }
 
 }  // namespace s2dw
