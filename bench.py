@@ -335,8 +335,8 @@ def bench_serve(args, precision=0):
 
 # stage name -> kernel whose launches dominate it
 STAGE_KERNEL = {"ana_contract": "k_ana_contract_mfma_w", "syn_contract": "k_syn_contract_mfma",
-                "ana_beta_fft": "k_beta_ana_t2", "ana_alpha_c2r": "k_alpha_c2r_t2",
-                "syn_alpha_r2c": "k_alpha_r2c_t2", "syn_beta_fft": "k_beta_syn_t2",
+                "ana_beta_fft": "k_beta_ana_h2", "ana_alpha_c2r": "k_alpha_c2r_o2",
+                "syn_alpha_r2c": "k_alpha_r2c_o2", "syn_beta_fft": "k_beta_syn_h2",
                 "ana_small_ffts": "k_alpha_c2r_g", "syn_small_ffts": "k_alpha_r2c_g"}
 
 

@@ -784,21 +784,24 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 // as it is accumulated, so a thread holds its R inputs and four
 // accumulators rather than 2R values.
 // ---------------------------------------------------------------------------
-template <int R, int DIR, typename Out>
-__device__ __forceinline__ void dft_odd_out(double2 *v, Out out) {
-  constexpr int H = (R - 1) / 2;
+template <int R, int DIR, int SPLIT = 1, typename Out>
+__device__ __forceinline__ void dft_odd_out(double2 *v, Out out, int part = 0) {
+  constexpr int H = (R - 1) / 2, QP = (H + SPLIT - 1) / SPLIT;
 #pragma unroll
   for (int r = 1; r <= H; ++r) {
     const double2 a = v[r], b = v[R - r];
     v[r] = cadd(a, b);
     v[R - r] = csub(a, b);
   }
-  double2 y0 = v[0];
+  if (part == 0) {
+    double2 y0 = v[0];
 #pragma unroll
-  for (int r = 1; r <= H; ++r) y0 = cadd(y0, v[r]);
-  out(0, y0);
+    for (int r = 1; r <= H; ++r) y0 = cadd(y0, v[r]);
+    out(0, y0);
+  }
 #pragma unroll
   for (int q = 1; q <= H; ++q) {
+    if (SPLIT > 1 && (q - 1) / QP != part) continue;  // part is uniform over the wave
     double re = v[0].x, im = v[0].y, ire = 0.0, iim = 0.0;
 #pragma unroll
     for (int r = 1; r <= H; ++r) {
@@ -816,15 +819,24 @@ __device__ __forceinline__ void dft_odd_out(double2 *v, Out out) {
   }
 }
 
-template <int R, int NN, int NS, int DIR, int NT, int T>
+// SPLIT > 1 (first stage only, NS = 1): the threads form SPLIT groups of
+// NT / SPLIT (whole waves); every group loads all butterflies and group p
+// forms the output pairs q of its share, which divides the stage's latency
+// (its longest chain of dependent instructions per lane) by SPLIT at the
+// price of SPLIT times the LDS reads.
+template <int R, int NN, int NS, int DIR, int NT, int T, int SPLIT = 1>
 __device__ __forceinline__ void odd_stage(double2 *buf, const double2 *__restrict__ tw) {
-  constexpr int nR = NN / R, nb = nR * T, NB = (nb + NT - 1) / NT, twstep = NN / (NS * R);
+  static_assert(SPLIT == 1 || NS == 1, "split stages carry no twiddles");
+  static_assert(NT % SPLIT == 0 && (NT / SPLIT) % 64 == 0, "split groups are whole waves");
+  constexpr int NG = NT / SPLIT;
+  constexpr int nR = NN / R, nb = nR * T, NB = (nb + NG - 1) / NG, twstep = NN / (NS * R);
   const int tid = stage_tid();
+  const int part = SPLIT > 1 ? __builtin_amdgcn_readfirstlane(tid / NG) : 0, lt = SPLIT > 1 ? tid % NG : tid;
   double2 v[NB][R];
   double2 w[NB][R];
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
-    const int b = tid + NT * q;
+    const int b = lt + NG * q;
     if (b < nb) {
       const int t = b % T, jj = b / T;
 #pragma unroll
@@ -839,7 +851,7 @@ __device__ __forceinline__ void odd_stage(double2 *buf, const double2 *__restric
   lds_barrier();
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
-    const int b = tid + NT * q;
+    const int b = lt + NG * q;
     if (b < nb) {
       const int t = b % T, jj = b / T;
       const int jm = NS > 1 ? jj % NS : 0;
@@ -848,16 +860,16 @@ __device__ __forceinline__ void odd_stage(double2 *buf, const double2 *__restric
         for (int r = 1; r < R; ++r) v[q][r] = cmulw<DIR>(v[q][r], w[q][r]);
       }
       const int o = ((jj - jm) * R + jm) * T + t;
-      dft_odd_out<R, DIR>(v[q], [&](int r, double2 y) { buf[o + r * NS * T] = y; });
+      dft_odd_out<R, DIR, SPLIT>(v[q], [&](int r, double2 y) { buf[o + r * NS * T] = y; }, part);
     }
   }
   lds_barrier();
 }
 
-template <int R0, int R1, int R2, int DIR, int NT, int T>
+template <int R0, int R1, int R2, int DIR, int NT, int T, int SPLIT = 1>
 __device__ __forceinline__ void odd_fft(double2 *buf, const double2 *__restrict__ tw) {
   constexpr int NN = R0 * R1 * R2;
-  odd_stage<R0, NN, 1, DIR, NT, T>(buf, tw);
+  odd_stage<R0, NN, 1, DIR, NT, T, SPLIT>(buf, tw);
   odd_stage<R1, NN, R0, DIR, NT, T>(buf, tw);
   odd_stage<R2, NN, R0 * R1, DIR, NT, T>(buf, tw);
 }
