@@ -3,9 +3,10 @@
 ! programs that use this library in place of the reference.
 !
 ! Public interface as the reference's module (src/mod/s2dw_fileio_mod.f90):
-! the generic s2dw_fileio_fits_wav_write / _read, each for statically
-! allocated coefficients wav(0:J, 0:2B-2, 0:2B-1, 0:N-1) and for the
-! per-scale type s2dw_wav_abg.  The file layout is the reference's, so files
+! the generics s2dw_fileio_fits_wav_write / _read and
+! s2dw_fileio_matlab_wav_write / _read, each for statically allocated
+! coefficients wav(0:J, 0:2B-2, 0:2B-1, 0:N-1) and for the per-scale type
+! s2dw_wav_abg.  The file layout is the reference's, so files
 ! are interchangeable both ways:
 !   HDU 1  empty primary image; comment cards, DATE, the optional comment and
 !          the keys J, B, N, BLSCOEFF (integers) and ALPHA (8 decimals)
@@ -17,8 +18,15 @@
 ! checks the number of HDUs, the scaling-coefficient sizes and for null
 ! values, as the reference does (1140-1240).  I/O goes through CFITSIO's
 ! Fortran interface (link with -lcfitsio).
-! The MATLAB text writers and readers of the reference (s2dw_fileio_mod.f90:
-! 102-763) are not provided.
+!
+! The MATLAB form (reference 102-763): a text file name.m, MATLAB code that
+! loads name_scoeff.dat (Re, then Im, of scoeff(l, m) for m <= l, one e27.20
+! value per line) and name_wcoeff.dat (every scale's (2hi-1) x 2hi x N cube,
+! first index fastest), preceded by a 19-line header comment and the lines
+! 'J = ...', 'N = ...', 'B = ...', 'bl_scoeff = ...', 'alpha = ...' and the
+! cube sizes that the readers parse.  Writing opens all three files as new
+! (an existing one is a Fortran I/O error); reading needs the layout exactly
+! as written.
 !------------------------------------------------------------------------------
 module s2dw_fileio_mod
 
@@ -31,8 +39,18 @@ module s2dw_fileio_mod
   private
 
   public :: &
+       s2dw_fileio_matlab_wav_write, &
+       s2dw_fileio_matlab_wav_read, &
        s2dw_fileio_fits_wav_write, &
        s2dw_fileio_fits_wav_read
+
+  interface s2dw_fileio_matlab_wav_write
+     module procedure matlab_write_static, matlab_write_dynamic
+  end interface s2dw_fileio_matlab_wav_write
+
+  interface s2dw_fileio_matlab_wav_read
+     module procedure matlab_read_static, matlab_read_dynamic
+  end interface s2dw_fileio_matlab_wav_read
 
   interface s2dw_fileio_fits_wav_write
      module procedure fits_write_static, fits_write_dynamic
@@ -309,5 +327,321 @@ contains
     end do
     call fits_finish(unit, status)
   end subroutine fits_read_dynamic
+
+  !----------------------------------------------------------------------------
+  ! MATLAB .m / .dat form
+  !----------------------------------------------------------------------------
+
+  ! name.m without its ".m"
+  function mat_base(filename) result(base)
+    character(len=*), intent(in) :: filename
+    character(len=STRING_LEN) :: base
+    base = filename(1:len_trim(filename) - 2)
+  end function mat_base
+
+  ! Header, parameters and cube sizes of name.m (the lines the readers count
+  ! and parse); sa, sb, sg: per-scale sizes.
+  subroutine mat_write_head(u, filename, J, N, B, bl_scoeff, alpha, sa, sb, sg, comment)
+    integer, intent(in) :: u, J, N, B, bl_scoeff
+    character(len=*), intent(in) :: filename
+    real(dp), intent(in) :: alpha
+    integer, intent(in) :: sa(0:J), sb(0:J), sg(0:J)
+    character(len=*), intent(in), optional :: comment
+    character(len=*), parameter :: sig = 'function [J, N, B, bl_scoeff, alpha, scoeff, wcoeff] = '
+    integer :: jj
+    write(u, '(a,a)') sig, trim(mat_base(filename))
+    write(u, '(a,a)') '% ' // sig, trim(mat_base(filename))
+    write(u, '(a)') '%'
+    write(u, '(a)') '%  Scale discretised wavelet coefficients created by S2DW code'
+    write(u, '(a)') '%  S2DW code written by Jason McEwen (mcewen@mrao.cam.ac.uk)'
+    write(u, '(a)') '%'
+    write(u, '(a)') '%  This file written by: Fortran'
+    if (present(comment)) then
+       write(u, '(a,a)') '%  Comment: ', trim(comment)
+    else
+       write(u, '(a)') '%  Comment: '
+    end if
+    write(u, '(a)') '%'
+    write(u, '(a)') '%  Variables:'
+    write(u, '(a)') '%   J: Maximum analysis scale depth.'
+    write(u, '(a)') '%   B: Harmonic band limit.'
+    write(u, '(a)') '%   N: Azimuthal band limit.'
+    write(u, '(a)') '%   bl_scoeff: Upper band limit for scaling coefficients.'
+    write(u, '(a)') '%   alpha: Basis dilation factor.'
+    write(u, '(a)') '%   scoeff(1:bl_scoeff,1:bl_scoeff): Scaling coefficients.'
+    write(u, '(a)') '%   wcoeff{1:J+1}(aa,bb,gg): Wavelet coefficients (ranges of aa, bb and gg, depend on scale).'
+    write(u, '(a)')
+    write(u, '(a)') '% Size parameters'
+    write(u, '(a,i27,a)') 'J         = ', J, ';'
+    write(u, '(a,i27,a)') 'N         = ', N, ';'
+    write(u, '(a,i27,a)') 'B         = ', B, ';'
+    write(u, '(a,i27,a)') 'bl_scoeff = ', bl_scoeff, ';'
+    write(u, '(a,e27.20,a)') 'alpha     = ', alpha, ';'
+    write(u, '(a)')
+    write(u, '(a)') '% Wavelet coefficient array sizes'
+    do jj = 0, J
+       write(u, '(a,i5,a,i15,a)') 'wcoeff_size_aa(', jj + 1, ') = ', sa(jj), ';'
+       write(u, '(a,i5,a,i15,a)') 'wcoeff_size_bb(', jj + 1, ') = ', sb(jj), ';'
+       write(u, '(a,i5,a,i15,a)') 'wcoeff_size_gg(', jj + 1, ') = ', sg(jj), ';'
+    end do
+    write(u, '(a)')
+  end subroutine mat_write_head
+
+  ! name_scoeff.dat and the .m lines that load it.
+  subroutine mat_write_scoeff(u, filename, scoeff, bl_scoeff)
+    integer, intent(in) :: u, bl_scoeff
+    character(len=*), intent(in) :: filename
+    complex(dpc), intent(in) :: scoeff(0:bl_scoeff-1, 0:bl_scoeff-1)
+    character(len=STRING_LEN) :: fdat
+    integer :: ud, el, m, part
+    write(fdat, '(a,a)') trim(mat_base(filename)), '_scoeff.dat'
+    open(newunit=ud, file=fdat, status='new', action='write', form='formatted')
+    do part = 0, 1
+       do el = 0, bl_scoeff - 1
+          do m = 0, el
+             if (part == 0) then
+                write(ud, '(e27.20)') real(scoeff(el, m), dp)
+             else
+                write(ud, '(e27.20)') real(aimag(scoeff(el, m)), dp)
+             end if
+          end do
+       end do
+    end do
+    close(ud)
+    write(u, '(a)') '% Scaling coefficients'
+    write(u, '(a,a,a)') 'scoeff_array = load(''', trim(fdat), ''');'
+    write(u, '(a)') 'n = length(scoeff_array) / 2;'
+    write(u, '(a)') 'scoeff_array = scoeff_array(1:n) + i*scoeff_array(n+1:end);'
+    write(u, '(a)') 'scoeff = zeros(bl_scoeff,bl_scoeff);'
+    write(u, '(a)') 'ind_start = 1;'
+    write(u, '(a)') 'for el = 0:bl_scoeff-1'
+    write(u, '(a)') '    ind_end = ind_start - 1 + el + 1;'
+    write(u, '(a)') '    scoeff(el+1, 1:el+1) = scoeff_array(ind_start:ind_end).'';'
+    write(u, '(a)') '    ind_start = ind_end + 1;'
+    write(u, '(a)') 'end'
+    write(u, '(a)')
+  end subroutine mat_write_scoeff
+
+  ! One scale's cube into the open name_wcoeff.dat (first index fastest).
+  subroutine mat_put_cube(ud, cube, sa, sb, sg)
+    integer, intent(in) :: ud, sa, sb, sg
+    real(dp), intent(in) :: cube(0:sa-1, 0:sb-1, 0:sg-1)
+    integer :: aa, bb, gg
+    do gg = 0, sg - 1
+       do bb = 0, sb - 1
+          do aa = 0, sa - 1
+             write(ud, '(e27.20)') cube(aa, bb, gg)
+          end do
+       end do
+    end do
+  end subroutine mat_put_cube
+
+  ! The .m lines that load name_wcoeff.dat.
+  subroutine mat_write_wload(u, fdat)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: fdat
+    write(u, '(a)') '% Wavelet coefficients'
+    write(u, '(a,a,a)') 'wcoeff_array = load(''', trim(fdat), ''');'
+    write(u, '(a)') 'ind_start = 1;'
+    write(u, '(a)') 'for jj = 1:J+1'
+    write(u, '(a)') '    ind_end = ind_start - 1 + wcoeff_size_aa(jj) * wcoeff_size_bb(jj) * wcoeff_size_gg(jj);'
+    write(u, '(a,a)') '    wcoeff{jj} = reshape(wcoeff_array(ind_start:ind_end),', &
+         'wcoeff_size_aa(jj), wcoeff_size_bb(jj), wcoeff_size_gg(jj));'
+    write(u, '(a)') '    ind_start = ind_end+1;'
+    write(u, '(a)') 'end'
+    write(u, '(a)')
+  end subroutine mat_write_wload
+
+  subroutine matlab_write_static(wav, scoeff, J, B, N, bl_scoeff, alpha, filename, comment)
+    integer, intent(in) :: J, B, N, bl_scoeff
+    real(dp), intent(in) :: alpha
+    real(dp), intent(in) :: wav(0:J, 0:2*B-2, 0:2*B-1, 0:N-1)
+    complex(dpc), intent(in) :: scoeff(0:bl_scoeff-1, 0:bl_scoeff-1)
+    character(len=*), intent(in) :: filename
+    character(len=*), intent(in), optional :: comment
+    integer :: u, ud, jj, hi, sa(0:J), sb(0:J), sg(0:J)
+    character(len=STRING_LEN) :: fdat
+    do jj = 0, J
+       hi = scale_hi(jj, B, alpha)
+       sa(jj) = 2 * hi - 1
+       sb(jj) = 2 * hi
+       sg(jj) = N
+    end do
+    open(newunit=u, file=trim(filename), status='new', action='write', form='formatted')
+    call mat_write_head(u, filename, J, N, B, bl_scoeff, alpha, sa, sb, sg, comment)
+    call mat_write_scoeff(u, filename, scoeff, bl_scoeff)
+    write(fdat, '(a,a)') trim(mat_base(filename)), '_wcoeff.dat'
+    open(newunit=ud, file=fdat, status='new', action='write', form='formatted')
+    do jj = 0, J
+       call mat_put_cube(ud, wav(jj, 0:sa(jj)-1, 0:sb(jj)-1, 0:N-1), sa(jj), sb(jj), N)
+    end do
+    close(ud)
+    call mat_write_wload(u, fdat)
+    close(u)
+  end subroutine matlab_write_static
+
+  subroutine matlab_write_dynamic(wavdyn, scoeff, J, B, N, bl_scoeff, alpha, filename, comment)
+    integer, intent(in) :: J, B, N, bl_scoeff
+    real(dp), intent(in) :: alpha
+    type(s2dw_wav_abg), intent(in) :: wavdyn(0:J)
+    complex(dpc), intent(in) :: scoeff(0:bl_scoeff-1, 0:bl_scoeff-1)
+    character(len=*), intent(in) :: filename
+    character(len=*), intent(in), optional :: comment
+    integer :: u, ud, jj, sa(0:J), sb(0:J), sg(0:J)
+    character(len=STRING_LEN) :: fdat
+    do jj = 0, J  ! sizes from the arrays themselves, as the reference does
+       sa(jj) = size(wavdyn(jj)%coeff, 1)
+       sb(jj) = size(wavdyn(jj)%coeff, 2)
+       sg(jj) = size(wavdyn(jj)%coeff, 3)
+    end do
+    open(newunit=u, file=trim(filename), status='new', action='write', form='formatted')
+    call mat_write_head(u, filename, J, N, B, bl_scoeff, alpha, sa, sb, sg, comment)
+    call mat_write_scoeff(u, filename, scoeff, bl_scoeff)
+    write(fdat, '(a,a)') trim(mat_base(filename)), '_wcoeff.dat'
+    open(newunit=ud, file=fdat, status='new', action='write', form='formatted')
+    do jj = 0, J
+       call mat_put_cube(ud, wavdyn(jj)%coeff, sa(jj), sb(jj), sg(jj))
+    end do
+    close(ud)
+    call mat_write_wload(u, fdat)
+    close(u)
+  end subroutine matlab_write_dynamic
+
+  ! Value after '=' and before the final ';' of a header line.
+  subroutine mat_value_line(u, text)
+    integer, intent(in) :: u
+    character(len=STRING_LEN), intent(out) :: text
+    character(len=STRING_LEN) :: line
+    read(u, '(a)') line
+    text = line(index(line, '=') + 1:len_trim(line) - 1)
+  end subroutine mat_value_line
+
+  ! Parameters and cube sizes from name.m.
+  subroutine mat_read_head(filename, J, N, B, bl_scoeff, alpha, sa, sb, sg)
+    character(len=*), intent(in) :: filename
+    integer, intent(out) :: J, N, B, bl_scoeff
+    real(dp), intent(out) :: alpha
+    integer, allocatable, intent(out) :: sa(:), sb(:), sg(:)
+    character(len=STRING_LEN) :: line
+    integer :: u, i, jj
+    open(newunit=u, file=trim(filename), status='old', action='read', form='formatted')
+    do i = 1, 19
+       read(u, '(a)') line
+    end do
+    call mat_value_line(u, line)
+    read(line, *) J
+    call mat_value_line(u, line)
+    read(line, *) N
+    call mat_value_line(u, line)
+    read(line, *) B
+    call mat_value_line(u, line)
+    read(line, *) bl_scoeff
+    call mat_value_line(u, line)
+    read(line, *) alpha
+    read(u, '(a)') line
+    read(u, '(a)') line
+    allocate(sa(0:J), sb(0:J), sg(0:J))
+    do jj = 0, J
+       call mat_value_line(u, line)
+       read(line, *) sa(jj)
+       call mat_value_line(u, line)
+       read(line, *) sb(jj)
+       call mat_value_line(u, line)
+       read(line, *) sg(jj)
+    end do
+    close(u)
+  end subroutine mat_read_head
+
+  ! scoeff(l, m), m <= l, from name_scoeff.dat (the rest zero).
+  subroutine mat_read_scoeff(filename, bl_scoeff, scoeff, proc)
+    character(len=*), intent(in) :: filename, proc
+    integer, intent(in) :: bl_scoeff
+    complex(dpc), allocatable, intent(out) :: scoeff(:,:)
+    real(dp) :: x
+    character(len=STRING_LEN) :: fdat
+    integer :: ud, el, m, fail
+    allocate(scoeff(0:bl_scoeff-1, 0:bl_scoeff-1), stat=fail)
+    if (fail /= 0) call s2dw_error(S2DW_ERROR_MEM_ALLOC_FAIL, proc)
+    scoeff = (0d0, 0d0)
+    write(fdat, '(a,a)') trim(mat_base(filename)), '_scoeff.dat'
+    open(newunit=ud, file=fdat, status='old', action='read', form='formatted')
+    do el = 0, bl_scoeff - 1
+       do m = 0, el
+          read(ud, '(e27.20)') x
+          scoeff(el, m) = cmplx(x, 0d0, dpc)
+       end do
+    end do
+    do el = 0, bl_scoeff - 1
+       do m = 0, el
+          read(ud, '(e27.20)') x
+          scoeff(el, m) = cmplx(real(scoeff(el, m), dp), x, dpc)
+       end do
+    end do
+    close(ud)
+  end subroutine mat_read_scoeff
+
+  subroutine mat_get_cube(ud, cube, sa, sb, sg)
+    integer, intent(in) :: ud, sa, sb, sg
+    real(dp), intent(out) :: cube(0:sa-1, 0:sb-1, 0:sg-1)
+    integer :: aa, bb, gg
+    do gg = 0, sg - 1
+       do bb = 0, sb - 1
+          do aa = 0, sa - 1
+             read(ud, '(e27.20)') cube(aa, bb, gg)
+          end do
+       end do
+    end do
+  end subroutine mat_get_cube
+
+  subroutine matlab_read_static(wav, scoeff, J, B, N, bl_scoeff, alpha, filename)
+    integer, intent(out) :: J, B, N, bl_scoeff
+    real(dp), intent(out) :: alpha
+    real(dp), allocatable, intent(out) :: wav(:,:,:,:)
+    complex(dpc), allocatable, intent(out) :: scoeff(:,:)
+    character(len=*), intent(in) :: filename
+    integer, allocatable :: sa(:), sb(:), sg(:)
+    real(dp), allocatable :: cube(:,:,:)
+    character(len=STRING_LEN) :: fdat
+    integer :: ud, jj, fail
+    call mat_read_head(filename, J, N, B, bl_scoeff, alpha, sa, sb, sg)
+    allocate(wav(0:J, 0:2*B-2, 0:2*B-1, 0:N-1), stat=fail)
+    if (fail /= 0) call s2dw_error(S2DW_ERROR_MEM_ALLOC_FAIL, 's2dw_fileio_matlab_wav_read_static')
+    wav = 0d0
+    call mat_read_scoeff(filename, bl_scoeff, scoeff, 's2dw_fileio_matlab_wav_read_static')
+    write(fdat, '(a,a)') trim(mat_base(filename)), '_wcoeff.dat'
+    open(newunit=ud, file=fdat, status='old', action='read', form='formatted')
+    do jj = 0, J
+       allocate(cube(0:sa(jj)-1, 0:sb(jj)-1, 0:sg(jj)-1))
+       call mat_get_cube(ud, cube, sa(jj), sb(jj), sg(jj))
+       wav(jj, 0:sa(jj)-1, 0:sb(jj)-1, 0:sg(jj)-1) = cube
+       deallocate(cube)
+    end do
+    close(ud)
+  end subroutine matlab_read_static
+
+  subroutine matlab_read_dynamic(wavdyn, scoeff, J, B, N, bl_scoeff, alpha, filename)
+    integer, intent(out) :: J, B, N, bl_scoeff
+    real(dp), intent(out) :: alpha
+    type(s2dw_wav_abg), allocatable, intent(out) :: wavdyn(:)
+    complex(dpc), allocatable, intent(out) :: scoeff(:,:)
+    character(len=*), intent(in) :: filename
+    integer, allocatable :: sa(:), sb(:), sg(:)
+    character(len=STRING_LEN) :: fdat
+    integer :: ud, jj, fail
+    call mat_read_head(filename, J, N, B, bl_scoeff, alpha, sa, sb, sg)
+    allocate(wavdyn(0:J), stat=fail)
+    if (fail /= 0) call s2dw_error(S2DW_ERROR_MEM_ALLOC_FAIL, 's2dw_fileio_matlab_wav_read_dynamic')
+    do jj = 0, J
+       allocate(wavdyn(jj)%coeff(0:sa(jj)-1, 0:sb(jj)-1, 0:sg(jj)-1), stat=fail)
+       if (fail /= 0) call s2dw_error(S2DW_ERROR_MEM_ALLOC_FAIL, 's2dw_fileio_matlab_wav_read_dynamic')
+    end do
+    call mat_read_scoeff(filename, bl_scoeff, scoeff, 's2dw_fileio_matlab_wav_read_dynamic')
+    write(fdat, '(a,a)') trim(mat_base(filename)), '_wcoeff.dat'
+    open(newunit=ud, file=fdat, status='old', action='read', form='formatted')
+    do jj = 0, J
+       call mat_get_cube(ud, wavdyn(jj)%coeff, sa(jj), sb(jj), sg(jj))
+    end do
+    close(ud)
+  end subroutine matlab_read_dynamic
 
 end module s2dw_fileio_mod

@@ -8,9 +8,11 @@ reference sources; the fixtures are committed, the GPU box only reads them.
 
     python tests/golden/gen_ref_fixtures.py [--moments-only | --fits-only]
 
-The FITS fixtures are wavelet files written by the reference's
-s2dw_fileio_fits_wav_write (oracle/_ref/ref_fits: s2dw_amd/fortran/
-s2dw_hip_fits_test.f90 linked against the reference's s2dw_fileio_mod).
+The FITS and MATLAB fixtures are wavelet files written by the reference's
+s2dw_fileio_fits_wav_write / s2dw_fileio_matlab_wav_write (oracle/_ref/
+ref_fits: s2dw_amd/fortran/s2dw_hip_fits_test.f90 linked against the
+reference's s2dw_fileio_mod); the MATLAB ones are written from this folder
+with relative names, which the .m file embeds.
 """
 from __future__ import annotations
 
@@ -27,6 +29,9 @@ REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
 REF_FITS = os.path.join(ROOT, "oracle", "_ref", "ref_fits")
 # (layout, B, N, alpha): the wavelet files written by the reference
 FITS_CASES = [("dynamic", 8, 2, 2.0), ("static", 12, 3, 3.0)]
+# (layout, B, N, alpha, J): MATLAB .m/.dat files written by the reference; J
+# below J_max so that there is more than one scaling coefficient
+MATLAB_CASES = [("dynamic", 8, 2, 2.0, 1), ("static", 6, 2, 3.0, 1)]
 
 # (B, alpha, J); J=None means J_max.  Covers the test config (64,2,full J),
 # L=256 N=5 full J, L=512 J_min=2 (J=7), alpha=3 and non-power-of-alpha B.
@@ -114,7 +119,18 @@ def fits_name(layout, B, N, alpha):
     return f"ref_wav_{layout}_B{B}_N{N}_a{int(alpha)}.fits"
 
 
+def matlab_name(layout, B, N, alpha, J):
+    return f"ref_mat_{layout}_B{B}_N{N}_a{int(alpha)}_J{J}"
+
+
 def fits_fixtures():
+    for layout, B, N, alpha, J in MATLAB_CASES:
+        base = matlab_name(layout, B, N, alpha, J)
+        for suffix in (".m", "_scoeff.dat", "_wcoeff.dat"):
+            if os.path.exists(os.path.join(HERE, base + suffix)):
+                os.remove(os.path.join(HERE, base + suffix))  # the writer refuses existing files
+        subprocess.run([REF_FITS, "mwrite_" + layout, base + ".m", str(B), str(N), str(alpha), str(J)], check=True,
+                       capture_output=True, cwd=HERE)
     for layout, B, N, alpha in FITS_CASES:
         out = os.path.join(HERE, fits_name(layout, B, N, alpha))
         if os.path.exists(out):
