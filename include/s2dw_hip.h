@@ -163,10 +163,13 @@ typedef struct s2dw_hip_plan *s2dw_hip_plan_t;
 /* Create a plan for (B, N, J, alpha) with the caller's K_gamma, Phi2, Slm
  * (layouts as above) and the chosen contraction precision.  All work of the
  * plan is enqueued on `stream` (a hipStream_t; NULL is the null stream).
- * Band limits up to B = 1024 are accepted; above that the alpha FFT of the
- * full-band scales (length 2B - 1) needs a Bluestein length of at least
- * 8192, whose two LDS buffers do not fit in the 160 KiB of LDS per
- * workgroup, and the call returns S2DW_ERROR_SIZE_INVALID. */
+ * Band limits up to B = 2048 are accepted (above B = 1024 the full-band
+ * scales' FFTs run through Bluestein on 8192 points in place, and the plan
+ * runs the streamed Wigner form whichever form the flags ask for; split
+ * precision is refused with S2DW_ERROR_ARG_INVALID); above
+ * that the alpha FFT (length 2B - 1) needs a Bluestein length of 16384,
+ * which does not fit in the 160 KiB of LDS per workgroup, and the call
+ * returns S2DW_ERROR_SIZE_INVALID. */
 int s2dw_hip_plan_create(s2dw_hip_plan_t *plan, int B, int N, int J, double alpha, const double *K_gamma,
                          const double *Phi2, const double *Slm, int precision, void *stream);
 int s2dw_hip_plan_destroy(s2dw_hip_plan_t plan);

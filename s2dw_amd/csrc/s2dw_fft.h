@@ -533,11 +533,15 @@ __device__ __forceinline__ void team_stage8(double2 *buf, int M, int logTw, int 
 
 // Power-of-two transform of all Tw sequences in the workgroup's slots, in
 // place: a first stage of radix R0 = 2^(log2 M mod 3) (no twiddles), then
-// radix-8 stages whose twiddles (at most 3 stages for M <= 2048) are loaded
-// once, up front, by team_twiddles.
-template <int E>
+// radix-8 stages whose twiddles (at most S = floor(log2(E NT) / 3) stages:
+// 3 for the 2048 slots of 256 threads, 4 for 8192 slots) are loaded once, up
+// front, by team_twiddles.
+__host__ __device__ constexpr int team_ilog2(int x) { return x <= 1 ? 0 : 1 + team_ilog2(x / 2); }
+template <int E, int NT>
+constexpr int team_s() { return team_ilog2(E * NT) / 3; }
+template <int E, int S>
 struct TeamTw {
-  double2 w[3][E / 8][3];
+  double2 w[S][E / 8][3];
 };
 
 __device__ __forceinline__ int ilog2_dev(int x) { return 31 - __clz(x); }
@@ -545,12 +549,12 @@ __device__ __forceinline__ int ilog2_dev(int x) { return 31 - __clz(x); }
 // ns0: Ns of the first radix-8 stage (1 << (log2 M mod 3) when the radix-2/4
 // stage comes first, 1 when it comes last)
 template <int E, int NT>
-__device__ __forceinline__ void team_twiddles(TeamTw<E> &T, int M, int logTw, const double2 *__restrict__ tw,
-                                              int ns0 = -1) {
+__device__ __forceinline__ void team_twiddles(TeamTw<E, team_s<E, NT>()> &T, int M, int logTw,
+                                              const double2 *__restrict__ tw, int ns0 = -1) {
   const int logM = ilog2_dev(M), nst = logM / 3;
   int Ns = ns0 > 0 ? ns0 : 1 << (logM % 3);
 #pragma unroll
-  for (int s = 0; s < 3; ++s) {
+  for (int s = 0; s < team_s<E, NT>(); ++s) {
     if (s < nst) {
       const int step = M / (Ns * 8);
 #pragma unroll
@@ -567,7 +571,7 @@ __device__ __forceinline__ void team_twiddles(TeamTw<E> &T, int M, int logTw, co
 }
 
 template <int E, int NT, int DIR>
-__device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const TeamTw<E> &T,
+__device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const TeamTw<E, team_s<E, NT>()> &T,
                                          const double2 *__restrict__ tw, const double2 *__restrict__ post,
                                          const double2 (&pm)[E / 8][8], bool use_pm) {
   static_assert(E % 8 == 0, "radix-8 engine");
@@ -576,7 +580,7 @@ __device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const T
   else if (r0 == 2) team_stage<4, E, NT, DIR>(buf, M, logTw, 1, tw, nst == 0 ? post : nullptr);
   int Ns = 1 << r0;
 #pragma unroll
-  for (int s = 0; s < 3; ++s) {
+  for (int s = 0; s < team_s<E, NT>(); ++s) {
     if (s < nst) {
       if (use_pm && s == nst - 1) team_stage8<E, NT, DIR, true>(buf, M, logTw, Ns, T.w[s], pm);
       else team_stage8<E, NT, DIR, false>(buf, M, logTw, Ns, T.w[s], pm);
@@ -603,11 +607,11 @@ __device__ __forceinline__ void team_fft(double2 *buf, int M, int logTw, const T
 template <int E, int NT, int DIR, bool FIRST_DONE, bool LAST_REGS>
 __device__ __forceinline__ void team_bluestein_x(const FftSpec &s, double2 *buf, int M, int logTw,
                                                  const double2 *__restrict__ post_last, double2 (&yl)[E / 4][2]) {
-  constexpr int NB = E / 8;
+  constexpr int NB = E / 8, S = team_s<E, NT>();
   const int logM = ilog2_dev(M), nst = logM / 3, r0 = logM % 3;
   const double2 *F = DIR > 0 ? s.filt_b : s.filt_f;
   const int Tw = 1 << logTw, nR = M >> 3;
-  TeamTw<E> T;
+  TeamTw<E, S> T;
   team_twiddles<E, NT>(T, M, logTw, s.tw);
   double2 pm[NB][8];
 #pragma unroll
@@ -623,7 +627,7 @@ __device__ __forceinline__ void team_bluestein_x(const FftSpec &s, double2 *buf,
   }
   int Ns = 1 << r0;
 #pragma unroll
-  for (int st = 0; st < 2; ++st) {
+  for (int st = 0; st < S - 1; ++st) {
     if (st < nst - 1) {
       team_stage8<E, NT, -1, false>(buf, M, logTw, Ns, T.w[st], pm);
       Ns *= 8;
@@ -644,17 +648,15 @@ __device__ __forceinline__ void team_bluestein_x(const FftSpec &s, double2 *buf,
 #pragma unroll
   for (int q = 0; q < NB; ++q)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) wl[q][i] = T.w[2][q][i];
-  if (nst == 2) {
+    for (int i = 0; i < 3; ++i) wl[q][i] = T.w[S - 1][q][i];
 #pragma unroll
-    for (int q = 0; q < NB; ++q)
+  for (int st = S - 2; st >= 0; --st) {
+    if (nst == st + 1) {
 #pragma unroll
-      for (int i = 0; i < 3; ++i) wl[q][i] = T.w[1][q][i];
-  } else if (nst == 1) {
+      for (int q = 0; q < NB; ++q)
 #pragma unroll
-    for (int q = 0; q < NB; ++q)
-#pragma unroll
-      for (int i = 0; i < 3; ++i) wl[q][i] = T.w[0][q][i];
+        for (int i = 0; i < 3; ++i) wl[q][i] = T.w[st][q][i];
+    }
   }
   team_twiddles<E, NT>(T, M, logTw, s.tw, 1);
   lds_barrier();  // every read of the forward transform is done
@@ -672,7 +674,7 @@ __device__ __forceinline__ void team_bluestein_x(const FftSpec &s, double2 *buf,
   lds_barrier();
   Ns = 8;
 #pragma unroll
-  for (int st = 1; st < 3; ++st) {
+  for (int st = 1; st < S; ++st) {
     if (st < nst) {
       team_stage8<E, NT, +1, false>(buf, M, logTw, Ns, T.w[st], pm);
       Ns *= 8;
@@ -729,7 +731,7 @@ __device__ __forceinline__ void team_fft_core(const FftSpec &s, double2 *buf, in
     team_bluestein<E, NT, DIR, FIRST_DONE>(s, buf, M, logTw, post_last);
     return;
   }
-  TeamTw<E> T;
+  TeamTw<E, team_s<E, NT>()> T;
   team_twiddles<E, NT>(T, M, logTw, s.tw);
   double2 pm[E / 8][8];
   if (!bluestein) {

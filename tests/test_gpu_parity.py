@@ -6,9 +6,10 @@ Tolerance of the FP64 plain form: 1e-12 max abs error on wavelet and scaling
 coefficients against the oracle and on the f_lm reconstruction.  The oracle
 restates the reference, including its double-precision Wigner planes, whose
 own rounding (up to ~1e-13 relative at the plane edge, pinned against exact
-values) reaches the wavelet coefficients.  The default (table) form reads
-planes generated on the device with that same rounding, bit for bit, and
-meets 1e-12 on every input; the streamed form (recursions in registers) is
+values) reaches the wavelet coefficients.  The host-buffer entry points use
+the reference-rounding form (the reference's planes, generated on the device
+bit for bit, its rounded phases and beta-sum basis) and meet 1e-12 on every
+input; device plans default to the streamed form (recursions in registers),
 held to an explicit, measured bound on white input.
 """
 import sys
@@ -379,11 +380,36 @@ def test_parity_largest_band_limits(B):
     assert e_syn <= TOL and e_rt <= TOL
 
 
-def test_band_limit_above_1024_refused():
-    """B = 1025 needs Bluestein at M = 8192, whose two LDS buffers (256 KiB)
-    exceed the 160 KiB of LDS per workgroup: the plan is refused with
-    SIZE_INVALID before any transform runs."""
-    B, N, alpha = 1025, 2, 2.0
+def test_parity_band_limit_above_1024():
+    """B = 1100: the full-band scales' alpha length 2B - 1 = 2199 and beta
+    length 2B = 2200 = 2^3 5^2 11 both run through Bluestein on 8192 points,
+    in place on workgroups of 1024 threads (128 KiB of LDS); scale 2 uses the
+    4096-point generic kernels.  The table forms need B <= 1024, so the
+    host-buffer entry points fall back to the streamed form here
+    (oracle: ~20 s on 16 cores)."""
+    B, N, alpha = 1100, 2, 2.0
+    say(f"parity B={B} N={N} full J")
+    st = setup(B, N, alpha)
+    f = seeded_flm(B, 4321 + B, red=True)
+    w_ref, sc_ref = st.analysis(f)
+    w, sc = core.s2dw_core_analysis_flm2wav_dynamic(f, st.K, st.S, st.J, B, N, st.bl_scoeff, alpha)
+    errs = [float(np.abs(a - b).max()) for a, b in zip(w, w_ref)]
+    say("  wav max abs err per scale", ["%.2e" % e for e in errs])
+    assert max(errs) <= TOL
+    np.testing.assert_array_equal(sc, sc_ref)
+    g = core.s2dw_core_synthesis_wav2flm_dynamic(w_ref, sc_ref, st.K, st.Phi2, st.S, st.J, B, N, st.bl_scoeff, alpha)
+    e_syn = float(np.abs(g - st.synthesis(w_ref, sc_ref)).max())
+    e_rt = float(np.abs(core.s2dw_core_synthesis_wav2flm_dynamic(w, sc, st.K, st.Phi2, st.S, st.J, B, N,
+                                                                  st.bl_scoeff, alpha) - f).max())
+    say(f"  synthesis vs oracle {e_syn:.2e}, round trip {e_rt:.2e}")
+    assert e_syn <= TOL and e_rt <= TOL
+
+
+def test_band_limit_above_2048_refused():
+    """B = 2049 needs Bluestein at M = 16384 (256 KiB in place), above the
+    160 KiB of LDS per workgroup: the plan is refused with SIZE_INVALID before
+    any transform runs."""
+    B, N, alpha = 2049, 1, 2.0
     say(f"refusal B={B}")
     J = core.s2dw_core_comp_Jmax(B, alpha)
     K, Phi2, bl = core.s2dw_core_init_kernels(J, B, alpha)
