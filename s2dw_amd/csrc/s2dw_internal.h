@@ -168,6 +168,7 @@ struct Plan {
   bool syn_wave = true;       // ... with independent waves (else a V tile staged in LDS)
   int smf_NCp = 0, smf_CW = 0, smf_nch = 0, n_swork = 0;
   int smf_wcount[3] = {0, 0, 0};  // work items per k-step class (<=2, <=4, <=8)
+  bool smf_side_small = false;     // classes 1, 2 read only scales j >= 2
   std::vector<int> smf_lb, smf_ub;
   std::vector<long long> smf_btoff;
   double *d_BT = nullptr;     // per-scale synthesis B tables
