@@ -473,9 +473,26 @@ def host_api_ms(J, K, Phi2, S, bl, flm):
         if i:
             ta.append(1e3 * (t1 - t0))
             ts.append(1e3 * (t2 - t1))
+    del w
+    # the static layout wav(0:J, 0:2B-2, 0:2B-1, 0:N-1): (J+1) x 1023 x 1024 x 8
+    # doubles (536 MB at this workload) each way, packed and unpacked on the device
+    sa, ss = [], []
+    for i in range(3):
+        t0 = time.perf_counter()
+        w, sc = core.s2dw_core_analysis_flm2wav(flm, K, S, J, L, N, bl, ALPHA)
+        t1 = time.perf_counter()
+        core.s2dw_core_synthesis_wav2flm(w, sc, K, Phi2, S, J, L, N, bl, ALPHA)
+        t2 = time.perf_counter()
+        if i:
+            sa.append(1e3 * (t1 - t0))
+            ss.append(1e3 * (t2 - t1))
+        del w
     return {"analysis_ms": statistics.median(ta), "synthesis_ms": statistics.median(ts),
+            "static_analysis_ms": statistics.median(sa), "static_synthesis_ms": statistics.median(ss),
+            "static_bytes": 8 * (J + 1) * (2 * L - 1) * (2 * L) * N,
             "note": "host-buffer entry points (the Fortran drop-in): reference-rounding form, f_lm and the "
-                    "wavelet coefficients copied over PCIe each way; median of 3 after a warm call"}
+                    "wavelet coefficients copied over PCIe each way; dynamic layout median of 3 after a warm "
+                    "call, static layout (jj fastest, packed on the device) median of 2 after a warm call"}
 
 
 def main():

@@ -6,6 +6,7 @@
 // built once, later calls with the same configuration only move data.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -60,6 +61,84 @@ struct HostEntry {
 std::mutex g_mu;
 std::list<std::unique_ptr<HostEntry>> g_cache;  // most recent first
 constexpr size_t kCacheMax = 4;
+
+// Copies between the caller's (pageable) arrays and the device.  Large ones
+// go through two pinned staging buffers: the DMA of one chunk overlaps the
+// host copy of the other, and host copies are split over threads.  The
+// buffers (2 x kStageBytes of pinned memory) are allocated on first use and
+// kept until the process ends; g_mu guards them.
+constexpr size_t kStageBytes = 32u << 20, kStageMin = 8u << 20, kPiece = 1u << 20;
+void *g_stage[2] = {nullptr, nullptr};
+hipEvent_t g_stage_ev[2] = {nullptr, nullptr};
+bool g_stage_rec[2] = {false, false};  // the event has been recorded
+
+int stage_ready() {
+  for (int i = 0; i < 2; ++i) {
+    if (!g_stage[i]) S2DW_HIP_CHECK(hipHostMalloc(&g_stage[i], kStageBytes, hipHostMallocDefault));
+    if (!g_stage_ev[i]) S2DW_HIP_CHECK(hipEventCreateWithFlags(&g_stage_ev[i], hipEventDisableTiming));
+    // a previous copy's transfers out of the buffer have finished
+    if (g_stage_rec[i]) S2DW_HIP_CHECK(hipEventSynchronize(g_stage_ev[i]));
+  }
+  return kErrNone;
+}
+
+int stage_record(int i, hipStream_t st) {
+  S2DW_HIP_CHECK(hipEventRecord(g_stage_ev[i], st));
+  g_stage_rec[i] = true;
+  return kErrNone;
+}
+
+void host_copy(void *dst, const void *src, size_t n) {
+  const long long np = (long long)((n + kPiece - 1) / kPiece);
+#pragma omp parallel for schedule(static) if (np > 1)
+  for (long long i = 0; i < np; ++i) {
+    const size_t o = (size_t)i * kPiece;
+    std::memcpy(static_cast<char *>(dst) + o, static_cast<const char *>(src) + o, std::min(kPiece, n - o));
+  }
+}
+
+// Device to host, in stream order after the work already on st; large
+// copies have completed on return, small ones at the next synchronisation.
+int copy_d2h(void *dst, const void *src, size_t n, hipStream_t st) {
+  if (n < kStageMin) {
+    S2DW_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, st));
+    return kErrNone;
+  }
+  if (int e = stage_ready()) return e;
+  const size_t nch = (n + kStageBytes - 1) / kStageBytes;
+  for (size_t i = 0; i <= nch; ++i) {
+    if (i < nch) {
+      const size_t o = i * kStageBytes, len = std::min(kStageBytes, n - o);
+      S2DW_HIP_CHECK(hipMemcpyAsync(g_stage[i & 1], static_cast<const char *>(src) + o, len, hipMemcpyDeviceToHost, st));
+      if (int e = stage_record(i & 1, st)) return e;
+    }
+    if (i > 0) {
+      const size_t k = i - 1, o = k * kStageBytes, len = std::min(kStageBytes, n - o);
+      S2DW_HIP_CHECK(hipEventSynchronize(g_stage_ev[k & 1]));
+      host_copy(static_cast<char *>(dst) + o, g_stage[k & 1], len);
+    }
+  }
+  return kErrNone;
+}
+
+// Host to device, in stream order on st; the source may be reused on return
+// (large copies have left it for the staging buffers).
+int copy_h2d(void *dst, const void *src, size_t n, hipStream_t st) {
+  if (n < kStageMin) {
+    S2DW_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+    return kErrNone;
+  }
+  if (int e = stage_ready()) return e;
+  const size_t nch = (n + kStageBytes - 1) / kStageBytes;
+  for (size_t i = 0; i < nch; ++i) {
+    const size_t o = i * kStageBytes, len = std::min(kStageBytes, n - o);
+    if (i >= 2) S2DW_HIP_CHECK(hipEventSynchronize(g_stage_ev[i & 1]));  // chunk i - 2 has left the buffer
+    host_copy(g_stage[i & 1], static_cast<const char *>(src) + o, len);
+    S2DW_HIP_CHECK(hipMemcpyAsync(static_cast<char *>(dst) + o, g_stage[i & 1], len, hipMemcpyHostToDevice, st));
+    if (int e = stage_record(i & 1, st)) return e;
+  }
+  return kErrNone;
+}
 
 // Rounding form of the host-buffer entry points: the reference's own
 // (default), or S2DW_HIP_ROUNDING=streamed / table for the other forms of
@@ -204,13 +283,12 @@ static int analysis_common(double *const *wavdyn, double *wav_static, double *sc
     if ((err = ensure_static(e))) return err;
     if ((err = pack_static(&p, e->d_wav, e->d_static, true))) return err;
     const size_t n = (size_t)(J + 1) * (2 * B - 1) * (2 * B) * N;
-    S2DW_HIP_CHECK(hipMemcpyAsync(wav_static, e->d_static, sizeof(double) * n, hipMemcpyDeviceToHost, p.stream));
+    if ((err = copy_d2h(wav_static, e->d_static, sizeof(double) * n, p.stream))) return err;
   } else {
     for (int j = 0; j <= J; ++j) {
       const Scale &s = p.sc[j];
       const size_t n = (size_t)(2 * s.hi - 1) * 2 * s.hi * N;
-      S2DW_HIP_CHECK(
-          hipMemcpyAsync(wavdyn[j], e->d_wav + s.wav_off, sizeof(double) * n, hipMemcpyDeviceToHost, p.stream));
+      if ((err = copy_d2h(wavdyn[j], e->d_wav + s.wav_off, sizeof(double) * n, p.stream))) return err;
     }
   }
   S2DW_HIP_CHECK(hipMemcpyAsync(scoeff, e->d_sc, sizeof(double2) * (size_t)bl_scoeff * bl_scoeff,
@@ -247,14 +325,13 @@ static int synthesis_common(double *flm, const double *const *wavdyn, const doub
   if (wav_static) {
     if ((err = ensure_static(e))) return err;
     const size_t n = (size_t)(J + 1) * (2 * B - 1) * (2 * B) * N;
-    S2DW_HIP_CHECK(hipMemcpyAsync(e->d_static, wav_static, sizeof(double) * n, hipMemcpyHostToDevice, p.stream));
+    if ((err = copy_h2d(e->d_static, wav_static, sizeof(double) * n, p.stream))) return err;
     if ((err = pack_static(&p, e->d_wav, e->d_static, false))) return err;
   } else {
     for (int j = 0; j <= J; ++j) {
       const Scale &s = p.sc[j];
       const size_t n = (size_t)(2 * s.hi - 1) * 2 * s.hi * N;
-      S2DW_HIP_CHECK(
-          hipMemcpyAsync(e->d_wav + s.wav_off, wavdyn[j], sizeof(double) * n, hipMemcpyHostToDevice, p.stream));
+      if ((err = copy_h2d(e->d_wav + s.wav_off, wavdyn[j], sizeof(double) * n, p.stream))) return err;
     }
   }
   S2DW_HIP_CHECK(hipMemcpyAsync(e->d_sc, scoeff, sizeof(double2) * (size_t)bl_scoeff * bl_scoeff,

@@ -259,6 +259,29 @@ def test_static_layout():
     assert np.abs(g - f).max() <= TOL
 
 
+def test_static_layout_matches_dynamic_B128():
+    """The static jj-fastest layout (s2dw_core_mod.f90:342-534, 816-1019) at
+    B=128: the pack and unpack only move values, so the static analysis
+    equals the dynamic one bit for bit inside each scale's block and is zero
+    outside it, and the static synthesis equals the dynamic synthesis bit for
+    bit."""
+    B, N, alpha = 128, 3, 2.0
+    say("static layout B=128 N=3 against the dynamic layout")
+    st = setup(B, N, alpha)
+    f = seeded_flm(B, 31)
+    ws, sc = core.s2dw_core_analysis_flm2wav(f, st.K, st.S, st.J, B, N, st.bl_scoeff, alpha)
+    wd, scd = core.s2dw_core_analysis_flm2wav_dynamic(f, st.K, st.S, st.J, B, N, st.bl_scoeff, alpha)
+    np.testing.assert_array_equal(sc, scd)
+    for j in range(st.J + 1):
+        h = st.hi[j]
+        np.testing.assert_array_equal(ws[j, :2 * h - 1, :2 * h, :], wd[j])
+        assert not ws[j, 2 * h - 1:, :, :].any() and not ws[j, :, 2 * h:, :].any()
+    g = core.s2dw_core_synthesis_wav2flm(ws, sc, st.K, st.Phi2, st.S, st.J, B, N, st.bl_scoeff, alpha)
+    gd = core.s2dw_core_synthesis_wav2flm_dynamic(wd, scd, st.K, st.Phi2, st.S, st.J, B, N, st.bl_scoeff, alpha)
+    np.testing.assert_array_equal(g, gd)
+    assert np.abs(g - f).max() <= TOL
+
+
 def test_device_plan_api_matches_host_api():
     import torch
 
