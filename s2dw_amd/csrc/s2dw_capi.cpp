@@ -548,19 +548,21 @@ int s2dw_hip_stat_moments(double *const *wavdyn, int J, int B, int N, double alp
   if (mask && (err = d.get(&d_m, (size_t)total))) return err;
   if ((err = d.get(&d_part, (size_t)moments_partial_doubles(cnt.data(), n)))) return err;
   if ((err = d.get(&d_out, 8 * (size_t)n))) return err;
-  for (int j = 0; j < n; ++j) {
-    S2DW_HIP_CHECK(hipMemcpyAsync(d_w + off[j], wavdyn[j], sizeof(double) * cnt[j], hipMemcpyHostToDevice, d.st));
-    if (mask)
-      S2DW_HIP_CHECK(hipMemcpyAsync(d_m + off[j], mask[j], sizeof(double) * cnt[j], hipMemcpyHostToDevice, d.st));
-  }
-  if ((err = moments_run(d.st, off.data(), cnt.data(), n, d_w, d_m, d_part, d_out))) return err;
   std::vector<double> out(8 * (size_t)n);
-  S2DW_HIP_CHECK(hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, d.st));
-  // with a mask the coefficients are returned masked (s2dw_stat_mod.f90:136-140)
-  if (mask)
-    for (int j = 0; j < n; ++j)
-      S2DW_HIP_CHECK(hipMemcpyAsync(wavdyn[j], d_w + off[j], sizeof(double) * cnt[j], hipMemcpyDeviceToHost, d.st));
-  S2DW_HIP_CHECK(hipStreamSynchronize(d.st));
+  {
+    std::lock_guard<std::mutex> lk(g_mu);  // the staging buffers
+    for (int j = 0; j < n; ++j) {
+      if ((err = copy_h2d(d_w + off[j], wavdyn[j], sizeof(double) * cnt[j], d.st))) return err;
+      if (mask && (err = copy_h2d(d_m + off[j], mask[j], sizeof(double) * cnt[j], d.st))) return err;
+    }
+    if ((err = moments_run(d.st, off.data(), cnt.data(), n, d_w, d_m, d_part, d_out))) return err;
+    S2DW_HIP_CHECK(hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, d.st));
+    // with a mask the coefficients are returned masked (s2dw_stat_mod.f90:136-140)
+    if (mask)
+      for (int j = 0; j < n; ++j)
+        if ((err = copy_d2h(wavdyn[j], d_w + off[j], sizeof(double) * cnt[j], d.st))) return err;
+    S2DW_HIP_CHECK(hipStreamSynchronize(d.st));
+  }
   int warn = -1;
   for (int j = 0; j < n; ++j) {
     if (out[7 * (size_t)n + j] == 2) {

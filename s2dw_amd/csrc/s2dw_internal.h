@@ -184,6 +184,7 @@ struct Plan {
   int fft_engine = 0;         // 0 workgroup engine, 1 wave engine, 2 generic Stockham
   bool odd_alpha = true;      // full-band alpha stages of 2hi - 1 = 1023 on the odd-length engine
   bool beta_one = true;       // full-band beta stages with one FFT per column (k_beta_*_h2)
+  bool syn_xcd = true;        // synthesis contraction: consecutive work items share an XCD's L2
   double2 *d_otw = nullptr;   // its twiddles e^{+2 pi i k/1023} (set when that engine runs)
   // HIP graphs of whole transforms for the device API: one executable graph
   // per (direction, buffers), captured on `cap` at first use and replayed
