@@ -19,6 +19,12 @@
 // share a hardware queue and a synthesis waiting for its analysis stalls
 // only its own lane.
 //
+// Each lane has a launcher thread: the worker admits a request (plan,
+// buffers, lane) and hands it to the lane's thread, which issues its copies
+// and kernels, so the launches of concurrent requests proceed in parallel
+// (a small transform's launch sequence costs more host time than its
+// kernels take on the device).
+//
 // The worker never blocks on a transfer: host data goes through pinned
 // staging buffers that the client threads fill (submit) and drain (wait),
 // so every copy the worker issues is asynchronous.  Device and pinned
@@ -104,8 +110,11 @@ struct Request {
   void *d_in = nullptr, *d_out = nullptr, *d_sc = nullptr;
   size_t n_in = 0, n_out = 0, n_sc = 0;
   hipEvent_t done = nullptr;
+  const void *src_wav = nullptr, *src_sc = nullptr;  // synthesis of a resident analysis: its outputs
+  hipEvent_t src_ready = nullptr;                    // and their completion event
+  bool issued = false;  // the lane thread has enqueued the work and recorded done
   int status = kErrNone;
-  std::string error;  // detail text of a failure (raised on the worker thread)
+  std::string error;  // detail text of a failure (raised on the thread that met it)
   bool finished = false;
   Clock::time_point t_submit, t_start, t_end;
 };
@@ -139,6 +148,10 @@ struct s2dw_hip_server {
   long long completed = 0;
   std::vector<hipStream_t> lanes;  // one stream per concurrent request
   std::vector<char> lane_busy;
+  std::vector<std::thread> lane_thr;                      // launcher thread per lane
+  std::vector<std::shared_ptr<s2dw::Request>> lane_job;   // request handed to the lane, until issued
+  std::condition_variable cv_lane;
+  bool stop_lanes = false;
   s2dw::Pool dpool;               // device buffers (counted in in_use)
   std::mutex pin_mu;              // guards hpool (client threads and the worker)
   s2dw::Pool hpool;               // pinned host buffers
@@ -328,6 +341,7 @@ SrcState source_of(s2dw_hip_server *s, const Request &r, const void **wav, const
   }
   for (auto &q : s->inflight)
     if (q->ticket == r.wav_from) {
+      if (!q->issued) return kSrcQueued;  // its completion event is not recorded yet
       if (q->status != kErrNone) return kSrcGone;
       *wav = q->d_out;
       *sc = q->d_sc;
@@ -435,33 +449,70 @@ bool admit_at(s2dw_hip_server *s, size_t qi, const void *src_wav, const void *sr
   r->n_out = n_out;
   r->n_sc = n_sc;
   if (!err) err = hchk(hipEventCreateWithFlags(&r->done, hipEventDisableTiming), "event");
-  const hipStream_t st = p.stream;
-  // every copy is between device and pinned memory: asynchronous
-  if (!err && r->kind == 0) {
-    err = hchk(hipMemcpyAsync(r->d_in, r->h_in.ptr, n_in, hipMemcpyHostToDevice, st), "copy in");
-    if (!err) err = run_analysis(&p, (double *)r->d_out, (double2 *)r->d_sc, (const double2 *)r->d_in);
-    if (!err && r->h_out.ptr)
-      err = hchk(hipMemcpyAsync(r->h_out.ptr, r->d_out, n_out, hipMemcpyDeviceToHost, st), "copy out");
-    if (!err && r->h_out2.ptr)
-      err = hchk(hipMemcpyAsync(r->h_out2.ptr, r->d_sc, n_sc, hipMemcpyDeviceToHost, st), "copy out");
-  } else if (!err) {
-    const void *wav = r->d_in, *sc = r->d_sc;
-    if (r->wav_from >= 0) {
-      wav = src_wav;
-      sc = src_sc;
-      err = hchk(hipStreamWaitEvent(st, src_ready, 0), "wait producer");
-    } else {
-      err = hchk(hipMemcpyAsync(r->d_in, r->h_in.ptr, n_in, hipMemcpyHostToDevice, st), "copy in");
-      if (!err) err = hchk(hipMemcpyAsync(r->d_sc, r->h_in2.ptr, n_sc, hipMemcpyHostToDevice, st), "copy in");
-    }
-    if (!err) err = run_synthesis(&p, (double2 *)r->d_out, (const double *)wav, (const double2 *)sc);
-    if (!err) err = hchk(hipMemcpyAsync(r->h_out.ptr, r->d_out, n_out, hipMemcpyDeviceToHost, st), "copy out");
+  r->src_wav = src_wav;
+  r->src_sc = src_sc;
+  r->src_ready = src_ready;
+  if (err) {
+    r->status = err;
+    r->error = s2dw_hip_last_error();
+    r->issued = true;  // nothing to issue
+  } else {
+    s->lane_job[r->lane] = r;  // the lane's thread issues the work
+    s->cv_lane.notify_all();
   }
-  if (!err) err = hchk(hipEventRecord(r->done, st), "event");
-  r->status = err;
-  if (err) r->error = s2dw_hip_last_error();
   s->inflight.push_back(r);
   return true;
+}
+
+// Enqueues a request's copies and kernels on its lane's stream and records
+// its completion event (on the lane's thread, without the server lock: the
+// request owns its plan, buffers and lane until it finishes).
+int issue(Request *r) {
+  Plan &p = r->plan->p;
+  const hipStream_t st = p.stream;
+  int err = kErrNone;
+  // every copy is between device and pinned memory: asynchronous
+  if (r->kind == 0) {
+    err = hchk(hipMemcpyAsync(r->d_in, r->h_in.ptr, r->n_in, hipMemcpyHostToDevice, st), "copy in");
+    if (!err) err = run_analysis(&p, (double *)r->d_out, (double2 *)r->d_sc, (const double2 *)r->d_in);
+    if (!err && r->h_out.ptr)
+      err = hchk(hipMemcpyAsync(r->h_out.ptr, r->d_out, r->n_out, hipMemcpyDeviceToHost, st), "copy out");
+    if (!err && r->h_out2.ptr)
+      err = hchk(hipMemcpyAsync(r->h_out2.ptr, r->d_sc, r->n_sc, hipMemcpyDeviceToHost, st), "copy out");
+  } else {
+    const void *wav = r->d_in, *sc = r->d_sc;
+    if (r->wav_from >= 0) {
+      wav = r->src_wav;
+      sc = r->src_sc;
+      err = hchk(hipStreamWaitEvent(st, r->src_ready, 0), "wait producer");
+    } else {
+      err = hchk(hipMemcpyAsync(r->d_in, r->h_in.ptr, r->n_in, hipMemcpyHostToDevice, st), "copy in");
+      if (!err) err = hchk(hipMemcpyAsync(r->d_sc, r->h_in2.ptr, r->n_sc, hipMemcpyHostToDevice, st), "copy in");
+    }
+    if (!err) err = run_synthesis(&p, (double2 *)r->d_out, (const double *)wav, (const double2 *)sc);
+    if (!err) err = hchk(hipMemcpyAsync(r->h_out.ptr, r->d_out, r->n_out, hipMemcpyDeviceToHost, st), "copy out");
+  }
+  if (!err) err = hchk(hipEventRecord(r->done, st), "event");
+  return err;
+}
+
+void lane_loop(s2dw_hip_server *s, int lane) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (true) {
+    s->cv_lane.wait(lk, [&] { return s->stop_lanes || s->lane_job[lane] != nullptr; });
+    if (!s->lane_job[lane]) break;  // stop_lanes and nothing handed over
+    std::shared_ptr<Request> r = s->lane_job[lane];
+    lk.unlock();
+    const int err = issue(r.get());
+    std::string msg = err ? s2dw_hip_last_error() : std::string();
+    lk.lock();
+    r->status = err;
+    if (err) r->error = msg;
+    r->issued = true;
+    s->lane_job[lane] = nullptr;
+    s->cv_work.notify_all();
+  }
 }
 
 void finish(s2dw_hip_server *s, const std::shared_ptr<Request> &r) {
@@ -508,7 +559,7 @@ void worker_loop(s2dw_hip_server *s) {
     // retire finished work
     for (size_t i = 0; i < s->inflight.size();) {
       auto &r = s->inflight[i];
-      const bool done = r->status != kErrNone || hipEventQuery(r->done) == hipSuccess;
+      const bool done = r->issued && (r->status != kErrNone || hipEventQuery(r->done) == hipSuccess);
       if (done) {
         finish(s, r);
         s->inflight.erase(s->inflight.begin() + (long)i);
@@ -614,6 +665,8 @@ int s2dw_hip_server_create_ex(s2dw_hip_server_t *srv, long long hbm_budget_bytes
     s->lanes.push_back(q);
     s->lane_busy.push_back(0);
   }
+  s->lane_job.assign(nl, nullptr);
+  for (int i = 0; i < nl; ++i) s->lane_thr.emplace_back(lane_loop, s, i);
   s->worker = std::thread(worker_loop, s);
   *srv = s;
   return kErrNone;
@@ -626,7 +679,13 @@ int s2dw_hip_server_destroy(s2dw_hip_server_t s) {
     s->stop = true;
     s->cv_work.notify_all();
   }
-  s->worker.join();
+  s->worker.join();  // the queue and the work in flight have drained
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop_lanes = true;
+    s->cv_lane.notify_all();
+  }
+  for (auto &t : s->lane_thr) t.join();
   for (auto &kv : s->resident) {
     (void)hipFree(kv.second.d_wav);
     (void)hipFree(kv.second.d_sc);
