@@ -185,6 +185,7 @@ struct Plan {
   bool odd_alpha = true;      // full-band alpha stages of 2hi - 1 = 1023 on the odd-length engine
   bool beta_one = true;       // full-band beta stages with one FFT per column (k_beta_*_h2)
   bool syn_xcd = true;        // synthesis contraction: consecutive work items share an XCD's L2
+  int syn_cw_min = 8;         // synthesis contraction: smallest m' chunk width
   double2 *d_otw = nullptr;   // its twiddles e^{+2 pi i k/1023} (set when that engine runs)
   // HIP graphs of whole transforms for the device API: one executable graph
   // per (direction, buffers), captured on `cap` at first use and replayed
