@@ -498,8 +498,10 @@ def host_api_ms(J, K, Phi2, S, bl, flm):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the GPU's clocks settle over tens of milliseconds of sustained work: the
+    # defaults time ~0.1 s of round trips after ~30 ms of warm-up
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run the byte-counter calibration copy and exit")
