@@ -201,7 +201,8 @@ def traffic_from(fetch_path, write_path):
     os.makedirs(os.path.dirname(TRAFFIC_JSON), exist_ok=True)
     with open(TRAFFIC_JSON, "w") as fh:
         json.dump({"source": "rocprofv3 --pmc FETCH_SIZE, then --pmc WRITE_SIZE, each over "
-                             "'python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline' and "
+                             "'python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-autotune --skip-forms "
+                             "--skip-serving --no-pipeline' and "
                              "'python3 bench.py --calib-copy'; per-dispatch KiB scaled by the counter's reading "
                              f"on the {CALIB_BYTES >> 20} MiB calibration copy", "kernels": out}, fh, indent=1)
     print(json.dumps(out, indent=1))
@@ -515,8 +516,8 @@ def main():
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of replaying "
                                                          "captured HIP graphs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel paths")
-    ap.add_argument("--pipeline", action="store_true", help="also measure round trips with the analysis of step "
-                                                            "i+1 overlapping the synthesis of step i")
+    ap.add_argument("--no-pipeline", action="store_true", help="skip the round trips with the analysis of step i+1 "
+                                                               "overlapping the synthesis of step i")
     ap.add_argument("--skip-forms", action="store_true", help="skip the split, table and reference-rounding forms")
     ap.add_argument("--skip-serving", action="store_true", help="skip the batch (64 transforms) and serving "
                                                                 "(Poisson stream) workloads")
@@ -575,7 +576,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
-    pipelined = bench_pipelined(args, L, N, J, K, Phi2, S, prec, d_f, flm) if args.pipeline else None
+    pipelined = None if args.no_pipeline else bench_pipelined(args, L, N, J, K, Phi2, S, prec, d_f, flm)
     # second pass: HIP events around every stage launch, on the plan's stream
     plan.set_timing(True)
     for i in range(steps):
