@@ -40,6 +40,7 @@ double quad_weight(double theta, int B);
 // Error text for the calling thread (s2dw_capi.cpp).
 void set_error(const char *fmt, ...);
 int hip_error(hipError_t e, const char *what);
+void host_copy_mt(void *dst, const void *src, size_t n);  // memcpy over OpenMP threads (large copies)
 
 #define S2DW_HIP_CHECK(call)                                  \
   do {                                                        \

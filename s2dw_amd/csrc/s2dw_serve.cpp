@@ -27,7 +27,8 @@
 //
 // The worker never blocks on a transfer: host data goes through pinned
 // staging buffers that the client threads fill (submit) and drain (wait),
-// so every copy the worker issues is asynchronous.  Device and pinned
+// so every copy the worker issues is asynchronous; the client threads split
+// large copies over threads.  Device and pinned
 // buffers are pooled by size; pooled device bytes count against the budget
 // and are the first thing freed when room is needed.
 #include <hip/hip_runtime.h>
@@ -604,14 +605,14 @@ int submit(s2dw_hip_server *s, std::shared_ptr<Request> r, long long *ticket) {
   if (r->kind == 0) {
     ok = pin_get(s, r->h_in, z.flm) && pin_get(s, r->h_out, r->out_host ? z.wav : 0) &&
          pin_get(s, r->h_out2, r->out_host2 ? z.sc : 0);
-    if (ok) std::memcpy(r->h_in.ptr, r->in_host, z.flm);
+    if (ok) host_copy_mt(r->h_in.ptr, r->in_host, z.flm);
   } else {
     ok = pin_get(s, r->h_out, z.flm);
     if (ok && r->wav_from < 0) {
       ok = pin_get(s, r->h_in, z.wav) && pin_get(s, r->h_in2, z.sc);
       if (ok) {
-        std::memcpy(r->h_in.ptr, r->in_host, z.wav);
-        std::memcpy(r->h_in2.ptr, r->in_host2, z.sc);
+        host_copy_mt(r->h_in.ptr, r->in_host, z.wav);
+        host_copy_mt(r->h_in2.ptr, r->in_host2, z.sc);
       }
     }
   }
@@ -739,8 +740,8 @@ int s2dw_hip_server_wait(s2dw_hip_server_t s, long long ticket, double *latency_
   }
   // outputs from the pinned staging buffers, on the caller's thread
   if (r->status == kErrNone) {
-    if (r->h_out.ptr && r->out_host) std::memcpy(r->out_host, r->h_out.ptr, r->h_out.n);
-    if (r->h_out2.ptr && r->out_host2) std::memcpy(r->out_host2, r->h_out2.ptr, r->h_out2.n);
+    if (r->h_out.ptr && r->out_host) host_copy_mt(r->out_host, r->h_out.ptr, r->h_out.n);
+    if (r->h_out2.ptr && r->out_host2) host_copy_mt(r->out_host2, r->h_out2.ptr, r->h_out2.n);
   }
   pin_release_all(s, *r);
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_end - r->t_submit).count();
