@@ -307,8 +307,8 @@ def test_device_plan_api_matches_host_api():
     plan.close()
 
 
-# B <= 128 plans run on one stream, B = 256 on two (fork and join captured)
-@pytest.mark.parametrize("B,N,stream", [(64, 5, "current"), (128, 3, "side"), (256, 3, "side")])
+# B <= 256 plans run on one stream, B = 512 on two (fork and join captured)
+@pytest.mark.parametrize("B,N,stream", [(64, 5, "current"), (128, 3, "side"), (512, 3, "side")])
 def test_graph_replay_matches_eager(B, N, stream):
     """Captured-graph replays, interleaved with eager work on the same plan and
     host synchronisations, give the eager results bit for bit on every replay
